@@ -1,0 +1,638 @@
+// The HiG-Flow projection step on one GPU.
+//
+// Setup (host): build the HiG-Tree domains exactly as the reference does
+// (higflow/src/hig-flow-kernel.c:389-507, 823-864; hig-flow-bc.c:27-122),
+// then record one step symbolically (see hfh_plan.hpp) into device operator
+// tables.  Step (device): interpolation rows -> fused per-facet momentum ->
+// cell divergence rows -> Krylov pressure solve -> fused projection rows and
+// pressure update; the viscoelastic step adds the velocity-gradient rows and
+// the fused per-cell log-conformation + polymeric-stress kernel.
+#pragma once
+#include <chrono>
+
+#include "hfh_kernels.hpp"
+#include "hfh_linsolve.hpp"
+#include "hfh_plan.hpp"
+#include "hfh_ve.hpp"
+
+namespace hfh {
+
+// Mirrors the fields of higflow_parameters/controllers/ve_parameters the
+// step reads (higflow/src/hig-flow-kernel.h:108-214, 425-482).
+struct Params {
+    double Re, dt, t;
+    int projtype, flowtype, tempdiscrtype, convecdiscrtype, secondconvecdiscrtype;
+    int ve_model, ve_convec, kernel_transform;
+    double De, beta, kernel_tol, giesekus_alpha;
+    double rtol, atol;
+    int maxit;
+    double pin_pressure;
+    int order;
+    int precision;      // 0 FP64 throughout, 1 mixed (FP32 inner Poisson in FP64 refinement)
+    double inner_rtol;  // mixed: FP32 inner tolerance
+};
+
+enum Stage { ST_INTERP = 0, ST_MOMENTUM, ST_DIV, ST_SOLVE, ST_PROJECT, ST_VE, ST_NSTAGE };
+
+struct BcSpec {
+    std::string amr;
+    int userid, ptype;
+    int utypes[3];
+};
+
+class FlowBase {
+  public:
+    virtual ~FlowBase() {}
+    virtual int dim() const = 0;
+    virtual int add_domain(const std::string &amr) = 0;
+    virtual int add_bc(const BcSpec &b) = 0;
+    virtual int setup(const Params &p) = 0;
+    virtual int field_size(int field, int comp) = 0;
+    virtual int field_centers(int field, int comp, double *xyz) = 0;
+    virtual int get_field(int field, int comp, double *out) = 0;
+    virtual int set_field(int field, int comp, const double *in) = 0;
+    virtual int bc_count(int kind, int i) = 0;
+    virtual int bc_centers(int kind, int i, double *xyz) = 0;
+    virtual int bc_set_values(int kind, int i, const double *vals) = 0;
+    virtual int step(bool ve) = 0;
+    virtual int stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn,
+                        int *ids, double *w, double *rhs) = 0;
+    virtual void set_time(double t) = 0;
+    virtual double time() const = 0;
+    virtual int pinned() const = 0;
+    virtual SolveStats last_solve() const = 0;
+    virtual void timing(double *ms_per_stage, long *calls) = 0;
+    virtual void reset_timing() = 0;
+    virtual long bytes(int stage) const = 0;
+    virtual void sync() = 0;
+};
+
+template <int D>
+class Flow : public FlowBase {
+  public:
+    static constexpr int NSYM = D * (D + 1) / 2;
+    // host_only: build the mesh, stencils and operator tables without a
+    // device (setup inspection and stencil queries); stepping then fails.
+    explicit Flow(bool host_only = false) : host_only_(host_only) {
+        if (!host_only_) HFH_CHECK(hipStreamCreate(&st_));
+    }
+    ~Flow() override {
+        if (host_only_) return;
+        for (int s = 0; s < ST_NSTAGE; s++) {
+            if (ev0_[s]) (void)hipEventDestroy(ev0_[s]);
+            if (ev1_[s]) (void)hipEventDestroy(ev1_[s]);
+        }
+        (void)hipStreamDestroy(st_);
+    }
+    int dim() const override { return D; }
+
+    int add_domain(const std::string &amr) override {
+        if (ready_) return -1;
+        auto t = std::make_unique<Tree<D>>();
+        if (!read_amr<D>(ids_, amr, *t)) return -2;
+        trees_.push_back(std::move(t));
+        return (int)trees_.size() - 1;
+    }
+
+    int add_bc(const BcSpec &b) override {
+        if (ready_) return -1;
+        specs_.push_back(b);
+        return (int)specs_.size() - 1;
+    }
+
+    int setup(const Params &p) override;
+    int field_size(int field, int comp) override;
+    int field_centers(int field, int comp, double *xyz) override;
+    int get_field(int field, int comp, double *out) override;
+    int set_field(int field, int comp, const double *in) override;
+    int bc_count(int kind, int i) override {
+        Boundary<D> *b = bc_of(kind, i);
+        return b ? (int)b->cells.size() : -1;
+    }
+    int bc_centers(int kind, int i, double *xyz) override {
+        Boundary<D> *b = bc_of(kind, i);
+        if (!b) return -1;
+        for (size_t k = 0; k < b->cells.size(); k++) {
+            Vec<D> c = b->tree->center(b->cells[k]);
+            for (int d = 0; d < D; d++) xyz[k * D + d] = c[d];
+        }
+        return (int)b->cells.size();
+    }
+    int bc_set_values(int kind, int i, const double *vals) override {
+        Boundary<D> *b = bc_of(kind, i);
+        if (!b) return -1;
+        for (size_t k = 0; k < b->cells.size(); k++) bcval_[b->base + k] = vals[k];
+        bc_dirty_ = true;
+        return (int)b->cells.size();
+    }
+    int step(bool ve) override;
+    int stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn, int *ids,
+                double *w, double *rhs) override;
+    void set_time(double t) override { par_.t = t; }
+    double time() const override { return par_.t; }
+    int pinned() const override { return desing_ ? pin_ : -1; }
+    SolveStats last_solve() const override { return stats_; }
+    void timing(double *ms, long *calls) override {
+        for (int s = 0; s < ST_NSTAGE; s++) {
+            ms[s] = ms_[s];
+            calls[s] = calls_[s];
+        }
+    }
+    void reset_timing() override {
+        for (int s = 0; s < ST_NSTAGE; s++) {
+            ms_[s] = 0;
+            calls_[s] = 0;
+        }
+    }
+    long bytes(int stage) const override { return stage >= 0 && stage < ST_NSTAGE ? bytes_[stage] : 0; }
+    void sync() override { HFH_CHECK(hipStreamSynchronize(st_)); }
+
+  private:
+    // ------------------------------------------------------------ host
+    bool host_only_ = false;
+    IdGen ids_;
+    std::vector<std::unique_ptr<Tree<D>>> trees_, bct_p_, bct_u_;
+    std::vector<BcSpec> specs_;
+    Params par_{};
+    bool ready_ = false, desing_ = true, bc_dirty_ = true;
+    int pin_ = -1;
+    std::unique_ptr<Domain<D>> sdp_, sdF_, sdED_, sfdu_[D], sfdF_[D];
+    std::vector<std::unique_ptr<Boundary<D>>> bcp_, bcu_[D];
+    std::vector<double> bcval_;
+
+    // state layout
+    int Nc_ = 0, Nf_[D] = {0};
+    int oU_[D], oUS_[D], oFU_[D], oP_, oDP_, oF_, oK_, oKN_, oS_, oDv_, oZero_, oPK_, oPcv_, oPmom_, nX_;
+    Group gDu_, gK_, gCv_, gMom_, gDiv_, gGrad_;
+    std::vector<int> momSlots_[D];
+    std::vector<unsigned> momFlags_[D];
+    std::vector<double> momDelta_[D];
+    int momNs_ = 0;
+    std::vector<int> veSlots_;  // per cell: D*(vl,vr) then D*(kl,kr,kll,krr)
+    std::vector<unsigned> veFlags_;
+    std::vector<double> veDelta_;
+    bool symmetric_ = true;
+    std::vector<int> Arp_, Aci_;
+    std::vector<double> Av_;
+
+    // ---------------------------------------------------------- device
+    hipStream_t st_ = nullptr;
+    DevBuf<double> X_, bvec_;
+    struct DGroup {
+        DevBuf<int> rp, col, out, y;
+        DevBuf<double> val, cst;
+        int nrow = 0, ncomp = 1, in_stride = 0, out_stride = 0;
+        bool has_y = false;
+    } dDu_, dK_, dCv_, dMom_, dDiv_, dGrad_;
+    DevBuf<int> dMomSlots_[D], dVeSlots_;
+    DevBuf<unsigned> dMomFlags_[D], dVeFlags_;
+    DevBuf<double> dMomDelta_[D], dVeDelta_;
+    DeviceSystem psys_;
+    SolveStats stats_;
+    hipEvent_t ev0_[ST_NSTAGE] = {}, ev1_[ST_NSTAGE] = {};
+    double ms_[ST_NSTAGE] = {0};
+    long calls_[ST_NSTAGE] = {0};
+    long bytes_[ST_NSTAGE] = {0};
+
+    Boundary<D> *bc_of(int kind, int i) {
+        if (!ready_ || i < 0 || i >= (int)specs_.size()) return nullptr;
+        if (kind == 0) return bcp_[i].get();
+        if (kind >= 1 && kind <= D) return bcu_[kind - 1][i].get();
+        return nullptr;
+    }
+
+    // ---------------------------------------------------------- record
+    static constexpr int UNUSED = -0x7fffffff;  // slot resolved to the zero entry
+    static int oZero_placeholder() { return UNUSED; }
+    static bool is_identity(const Stencil &s) {
+        return s.ids.size() == 1 && s.w[0] == 1.0 && s.bc.empty() && s.rhs0 == 0.0;
+    }
+    Stencil stencil_at(Domain<D> &dom, const Vec<D> &c, const Vec<D> &x) {
+        Stencil s;
+        dom.get_stencil(c, x, 1.0, s);
+        return s;
+    }
+    // slot for a point value: state offset, or -(1+row) of group g
+    int point_slot(Domain<D> &dom, int field, const Vec<D> &c, const Vec<D> &x, Group &g, bool force_row = false) {
+        Stencil s = stencil_at(dom, c, x);
+        if (!force_row && is_identity(s)) return field + s.ids[0];
+        auto k = point_key<D>(x);
+        char key[96];
+        std::snprintf(key, sizeof key, "%p:%d:%lld:%lld:%lld", (void *)&dom, field, (long long)k[0],
+                      (long long)k[1], (long long)k[2]);
+        auto it = g.dedup.find(key);
+        if (it != g.dedup.end()) return -(1 + it->second);
+        Combo cb;
+        cb.add(s, field, 1.0);
+        int r = g.add_row(cb.cols, cb.rhs0, cb.bc, -1);
+        g.dedup[key] = r;
+        return -(1 + r);
+    }
+    int combo_row(Group &g, const Combo &cb, int out) { return g.add_row(cb.cols, cb.rhs0, cb.bc, out); }
+
+    void record();
+    void record_momentum(bool ve);
+    void record_pressure();
+    void record_projection();
+    void record_ve();
+    void finalize_layout();
+    void validate() const;
+    void upload_group(DGroup &dg, const Group &g, int pbase, int out_stride, bool has_y);
+    void refold_constants();
+    void run_group(DGroup &dg, double alpha, double *dst, int stage);
+    void tbegin(int s) {
+        if (!ev0_[s]) {
+            HFH_CHECK(hipEventCreate(&ev0_[s]));
+            HFH_CHECK(hipEventCreate(&ev1_[s]));
+        }
+        HFH_CHECK(hipEventRecord(ev0_[s], st_));
+    }
+    void tend(int s) {
+        HFH_CHECK(hipEventRecord(ev1_[s], st_));
+        HFH_CHECK(hipEventSynchronize(ev1_[s]));
+        float m = 0;
+        HFH_CHECK(hipEventElapsedTime(&m, ev0_[s], ev1_[s]));
+        ms_[s] += m;
+        calls_[s]++;
+    }
+};
+
+// ================================================================= setup
+template <int D>
+int Flow<D>::setup(const Params &p) {
+    if (ready_ || trees_.empty()) return -1;
+    par_ = p;
+    if (par_.order <= 0) par_.order = 2;
+    if (par_.maxit <= 0) par_.maxit = 20000;
+    if (par_.inner_rtol <= 0) par_.inner_rtol = 1e-5;
+    int order = par_.order;
+    std::vector<const Tree<D> *> ct;
+    std::vector<Tree<D> *> mt;
+    for (auto &t : trees_) {
+        ct.push_back(t.get());
+        mt.push_back(t.get());
+    }
+    sdp_.reset(new Domain<D>(order, -1));
+    sdF_.reset(new Domain<D>(order, -1));
+    sdED_.reset(new Domain<D>(order, -1));
+    for (auto *d : {sdp_.get(), sdF_.get(), sdED_.get()}) d->trees = ct;
+    for (int d = 0; d < D; d++) {
+        sfdu_[d].reset(new Domain<D>(order, d));
+        sfdF_[d].reset(new Domain<D>(order, d));
+        sfdu_[d]->trees = ct;
+        sfdF_[d]->trees = ct;
+        Domain<D>::adjust_facet_ids(mt, d);
+        Domain<D>::adjust_facet_ids(mt, d);
+    }
+    for (auto &s : specs_) {
+        auto tp = std::make_unique<Tree<D>>(), tu = std::make_unique<Tree<D>>();
+        if (!read_amr<D>(ids_, s.amr, *tp) || !read_amr<D>(ids_, s.amr, *tu)) return -2;
+        bct_p_.push_back(std::move(tp));
+        bct_u_.push_back(std::move(tu));
+    }
+    int maxid = ids_.next + 1;
+    sdp_->build_cell_map(maxid);
+    sdF_->build_cell_map(maxid);
+    sdED_->build_cell_map(maxid);
+    for (int d = 0; d < D; d++) {
+        sfdu_[d]->compute_sfbi();
+        sfdu_[d]->build_facet_map(maxid);
+        sfdF_[d]->compute_sfbi();
+        sfdF_[d]->build_facet_map(maxid);
+    }
+    int base = 0;
+    desing_ = true;
+    for (size_t i = 0; i < specs_.size(); i++) {
+        auto b = std::make_unique<Boundary<D>>();
+        b->tree = bct_p_[i].get();
+        b->type = specs_[i].ptype;
+        b->userid = specs_[i].userid;
+        b->build(maxid);
+        b->base = base;
+        base += (int)b->cells.size();
+        if (b->type == DIRICHLET) {
+            sdp_->dir.push_back(b.get());
+            desing_ = false;
+        } else {
+            sdp_->neu.push_back(b.get());
+        }
+        bcp_.push_back(std::move(b));
+    }
+    for (int d = 0; d < D; d++)
+        for (size_t i = 0; i < specs_.size(); i++) {
+            auto b = std::make_unique<Boundary<D>>();
+            b->tree = bct_u_[i].get();
+            b->type = specs_[i].utypes[d];
+            b->userid = specs_[i].userid;
+            b->build(maxid);
+            b->base = base;
+            base += (int)b->cells.size();
+            if (b->type == DIRICHLET) sfdu_[d]->dir.push_back(b.get());
+            else sfdu_[d]->neu.push_back(b.get());
+            bcu_[d].push_back(std::move(b));
+        }
+    bcval_.assign(base + 1, 0.0);
+    Nc_ = sdp_->size();
+    for (int d = 0; d < D; d++) Nf_[d] = sfdu_[d]->size();
+    {
+        const Tree<D> &t0 = *trees_[0];
+        int c2 = t0.cell_with_point(t0.center(0));
+        pin_ = sdp_->map.lookup(t0.n[c2].cid);  // hig-flow-step.c:22-28
+    }
+    // state layout
+    int o = 0;
+    for (int d = 0; d < D; d++) {
+        oU_[d] = o;
+        o += Nf_[d];
+    }
+    for (int d = 0; d < D; d++) {
+        oUS_[d] = o;
+        o += Nf_[d];
+    }
+    for (int d = 0; d < D; d++) {
+        oFU_[d] = o;
+        o += sfdF_[d]->size();
+    }
+    oP_ = o;
+    o += Nc_;
+    oDP_ = o;
+    o += Nc_;
+    oF_ = o;
+    o += sdF_->size();
+    int ne = sdED_->size();
+    oK_ = o;
+    o += NSYM * ne;
+    oKN_ = o;
+    o += NSYM * ne;
+    oS_ = o;
+    o += D * D * ne;
+    oDv_ = o;
+    o += D * D * ne;
+    nX_ = o;  // P regions appended by finalize_layout
+    ready_ = true;
+    record();
+    finalize_layout();
+    return 0;
+}
+
+template <int D>
+void Flow<D>::record() {
+    bool ve = par_.flowtype == VISCOELASTIC;
+    if (ve) record_ve();
+    record_momentum(ve);
+    record_pressure();
+    record_projection();
+}
+
+// hig-flow-discret.c:86-259 (+541-569 for the tensor divergence), in the
+// same evaluation order as the reference.
+template <int D>
+void Flow<D>::record_momentum(bool ve) {
+    momNs_ = 2 + 6 * D + (D - 1) + (ve ? D : 0);
+    const int ns = momNs_;
+    for (int dim = 0; dim < D; dim++) {
+        Domain<D> &fu = *sfdu_[dim];
+        int nf = Nf_[dim];
+        momSlots_[dim].assign((size_t)ns * nf, UNUSED);
+        momFlags_[dim].assign(nf, 0);
+        momDelta_[dim].assign((size_t)D * nf, 0.0);
+        for (int f = 0; f < nf; f++) {
+            Vec<D> fc = fu.center_of(f), fd = fu.delta_of(f);
+            for (int k = 0; k < D; k++) momDelta_[dim][(size_t)k * nf + f] = fd[k];
+            auto S = [&](int k) -> int & { return momSlots_[dim][(size_t)k * nf + f]; };
+            int ctype = par_.convecdiscrtype;
+            // F = FU(fc) + (F(fc-) + F(fc+))/2
+            {
+                Combo cb;
+                cb.add(stencil_at(*sfdF_[dim], fc, fc), oFU_[dim], 1.0);
+                Vec<D> pl = fc, pr = fc;
+                pl[dim] = fc[dim] - 0.5 * fd[dim];
+                pr[dim] = fc[dim] + 0.5 * fd[dim];
+                // pressure points are evaluated between the two source reads
+                Stencil sl = stencil_at(*sdp_, fc, pl);
+                Stencil sr = stencil_at(*sdp_, fc, pr);
+                Stencil fl = stencil_at(*sdF_, fc, pl);
+                Stencil fr = stencil_at(*sdF_, fc, pr);
+                cb.add(fl, oF_, 0.5);
+                cb.add(fr, oF_, 0.5);
+                S(0) = -(1 + combo_row(gMom_, cb, -1));
+                Combo cp;
+                double k = 1.0 / (2.0 * 0.5 * fd[dim]);
+                cp.add(sr, oP_, k);
+                cp.add(sl, oP_, -k);
+                S(1) = -(1 + combo_row(gMom_, cp, -1));
+            }
+            unsigned flags = 0;
+            int vk = 2 + 6 * D;
+            for (int d2 = 0; d2 < D; d2++) {
+                const int b = 2 + 6 * d2;
+                if (d2 != dim) {  // hig-flow-eval.c:60-92
+                    Combo cv;
+                    double sg[4][2] = {{+1, +1}, {+1, -1}, {-1, +1}, {-1, -1}};
+                    for (int q = 0; q < 4; q++) {
+                        Vec<D> p = fc;
+                        p[dim] = fc[dim] + sg[q][0] * (0.5 * 1.0 * fd[dim]);
+                        p[d2] = fc[d2] + sg[q][1] * (0.5 * 1.0 * fd[d2]);
+                        if (sg[q][0] < 0) p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
+                        else p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
+                        if (sg[q][1] < 0) p[d2] = fc[d2] - 0.5 * 1.0 * fd[d2];
+                        else p[d2] = fc[d2] + 0.5 * 1.0 * fd[d2];
+                        cv.add(stencil_at(*sfdu_[d2], fc, p), oU_[d2], 0.25);
+                    }
+                    S(vk++) = -(1 + combo_row(gMom_, cv, -1));
+                }
+                Vec<D> xl = fc, xr = fc, xll = fc, xrr = fc;
+                xl[d2] = fc[d2] - 1.0 * fd[d2];
+                xr[d2] = fc[d2] + 1.0 * fd[d2];
+                xrr[d2] = fc[d2] + 2.0 * fd[d2];
+                xll[d2] = fc[d2] - 2.0 * fd[d2];
+                S(b + 0) = point_slot(fu, oU_[dim], fc, xl, gMom_);
+                if (!fu.facet_with_point(xl) && ctype == SECOND_ORDER) ctype = FIRST_ORDER;
+                S(b + 1) = point_slot(fu, oU_[dim], fc, xr, gMom_);
+                if (!fu.facet_with_point(xr) && ctype == SECOND_ORDER) ctype = FIRST_ORDER;
+                S(b + 2) = point_slot(fu, oU_[dim], fc, xrr, gMom_);
+                bool irr = fu.facet_with_point(xrr);
+                S(b + 3) = point_slot(fu, oU_[dim], fc, xll, gMom_);
+                bool ill = fu.facet_with_point(xll);
+                if (irr) flags |= 1u << (2 + d2);
+                if (ill) flags |= 1u << (2 + D + d2);
+                if (ctype == SECOND_ORDER && par_.secondconvecdiscrtype == CUBISTA) {
+                    (void)point_slot(fu, oU_[dim], fc, xl, gMom_);
+                    (void)point_slot(fu, oU_[dim], fc, xll, gMom_);
+                    if (!fu.facet_with_point(xl)) ctype = FIRST_ORDER;
+                    (void)point_slot(fu, oU_[dim], fc, xr, gMom_);
+                    (void)point_slot(fu, oU_[dim], fc, xrr, gMom_);
+                    if (!fu.facet_with_point(xr)) ctype = FIRST_ORDER;
+                    Vec<D> pr = fc, pl = fc;
+                    pr[d2] = fc[d2] + 0.5 * fd[d2];
+                    pl[d2] = fc[d2] - 0.5 * fd[d2];
+                    S(b + 4) = point_slot(*sfdu_[d2], oU_[d2], fc, pr, gMom_);
+                    S(b + 5) = point_slot(*sfdu_[d2], oU_[d2], fc, pl, gMom_);
+                }
+            }
+            flags |= (unsigned)ctype & 3u;
+            momFlags_[dim][f] = flags;
+            if (ve) {  // hig-flow-discret.c:541-569
+                int tk = 2 + 6 * D + (D - 1);
+                for (int d2 = 0; d2 < D; d2++) {
+                    int fS = oS_ + (dim * D + d2) * sdED_->size();
+                    Combo cs;
+                    if (d2 == dim) {
+                        Vec<D> pl = fc, pr = fc;
+                        pl[d2] = fc[d2] - 0.5 * fd[d2];
+                        pr[d2] = fc[d2] + 0.5 * fd[d2];
+                        Stencil a = stencil_at(*sdED_, fc, pl), bb = stencil_at(*sdED_, fc, pr);
+                        double k = 1.0 / (2.0 * 0.5 * fd[d2]);
+                        cs.add(bb, fS, k);
+                        cs.add(a, fS, -k);
+                    } else {
+                        Stencil l1, l2, r1, r2;
+                        Vec<D> p = fc;
+                        p[d2] = fc[d2] - 1.0 * fd[d2];
+                        p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
+                        l1 = stencil_at(*sdED_, fc, p);
+                        p[d2] = fc[d2] - 1.0 * fd[d2];
+                        p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
+                        l2 = stencil_at(*sdED_, fc, p);
+                        p[d2] = fc[d2] + 1.0 * fd[d2];
+                        p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
+                        r1 = stencil_at(*sdED_, fc, p);
+                        p[d2] = fc[d2] + 1.0 * fd[d2];
+                        p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
+                        r2 = stencil_at(*sdED_, fc, p);
+                        double k = 1.0 / (2.0 * 1.0 * fd[d2]);
+                        cs.add(r1, fS, 0.5 * k);
+                        cs.add(r2, fS, 0.5 * k);
+                        cs.add(l1, fS, -0.5 * k);
+                        cs.add(l2, fS, -0.5 * k);
+                    }
+                    S(tk + d2) = -(1 + combo_row(gMom_, cs, -1));
+                }
+            }
+        }
+    }
+}
+
+// hig-flow-step.c:48-131 with the pinned unknown eliminated symmetrically
+template <int D>
+void Flow<D>::record_pressure() {
+    Domain<D> &sp = *sdp_;
+    int n = Nc_;
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+    std::vector<Combo> bcombo(n);
+    for (int c = 0; c < n; c++) {
+        Vec<D> cc = sp.center_of(c), cd = sp.delta_of(c);
+        Combo &cb = bcombo[c];
+        for (int dim = 0; dim < D; dim++) {
+            Vec<D> pl = cc, pr = cc;
+            pl[dim] = cc[dim] - 0.5 * cd[dim];
+            pr[dim] = cc[dim] + 0.5 * cd[dim];
+            Stencil sl = stencil_at(*sfdu_[dim], cc, pl), sr = stencil_at(*sfdu_[dim], cc, pr);
+            double k = 1.0 / (2.0 * 0.5 * cd[dim]);
+            cb.add(sr, oUS_[dim], k);
+            cb.add(sl, oUS_[dim], -k);
+        }
+        cb.scale(1.0 / par_.dt);
+        Stencil L;
+        double alpha = 0.0;
+        for (int dim = 0; dim < D; dim++) {
+            double w = 1.0 / (cd[dim] * cd[dim]);
+            alpha -= 2.0 * w;
+            Vec<D> p = cc;
+            p[dim] = cc[dim] + cd[dim];
+            sp.get_stencil(cc, p, w, L);
+            p[dim] = cc[dim] - cd[dim];
+            sp.get_stencil(cc, p, w, L);
+        }
+        sp.get_stencil(cc, cc, alpha, L);
+        // b = div/dt + rhs(L): fold rhs(L) with opposite sign (value = sum - rhs)
+        cb.rhs0 -= L.rhs0;
+        for (auto &bb : L.bc) cb.bc.emplace_back(bb.first, -bb.second);
+        for (size_t k = 0; k < L.ids.size(); k++) rows[c].emplace_back(L.ids[k], L.w[k]);
+    }
+    // pinned row (solver.c:142-149): x_pin = p_exact - p[pin] (incremental)
+    bool incr = par_.projtype == INCREMENTAL;
+    std::vector<double> colpin(n, 0.0);
+    if (desing_) {
+        for (int c = 0; c < n; c++) {
+            if (c == pin_) continue;
+            for (auto &e : rows[c])
+                if (e.first == pin_) colpin[c] += e.second;
+        }
+    }
+    symmetric_ = true;
+    std::vector<std::map<int, double>> M(n);
+    for (int c = 0; c < n; c++) {
+        if (desing_ && c == pin_) {
+            M[c][c] = 1.0;
+            continue;
+        }
+        for (auto &e : rows[c]) {
+            if (desing_ && e.first == pin_) continue;
+            M[c][e.first] += -e.second;
+        }
+    }
+    for (int c = 0; c < n && symmetric_; c++)
+        for (auto &e : M[c]) {
+            auto it = M[e.first].find(c);
+            double o = it == M[e.first].end() ? 0.0 : it->second;
+            if (std::fabs(o - e.second) > 1e-12 * std::max(1.0, std::fabs(e.second))) {
+                symmetric_ = false;
+                break;
+            }
+        }
+    Arp_.assign(1, 0);
+    Aci_.clear();
+    Av_.clear();
+    for (int c = 0; c < n; c++) {
+        for (auto &e : M[c]) {
+            Aci_.push_back(e.first);
+            Av_.push_back(e.second);
+        }
+        Arp_.push_back((int)Aci_.size());
+    }
+    for (int c = 0; c < n; c++) {
+        Combo cb = bcombo[c];
+        if (desing_ && c == pin_) {
+            Combo pc;
+            pc.rhs0 = -par_.pin_pressure;  // value = pin_p - p[pin]
+            if (incr) pc.add_state(oP_ + pin_, -1.0);
+            combo_row(gDiv_, pc, c);
+            continue;
+        }
+        if (desing_ && colpin[c] != 0.0) {
+            // b_c - A_c,pin * x_pin
+            cb.rhs0 += colpin[c] * par_.pin_pressure;
+            if (incr) cb.add_state(oP_ + pin_, colpin[c]);
+        }
+        cb.scale(-1.0);
+        combo_row(gDiv_, cb, c);
+    }
+}
+
+// hig-flow-step.c:242-308
+template <int D>
+void Flow<D>::record_projection() {
+    bool incr = par_.projtype == INCREMENTAL;
+    int src = incr ? oDP_ : oP_;
+    for (int dim = 0; dim < D; dim++) {
+        Domain<D> &fu = *sfdu_[dim];
+        for (int f = 0; f < Nf_[dim]; f++) {
+            Vec<D> fc = fu.center_of(f), fd = fu.delta_of(f);
+            Vec<D> pl = fc, pr = fc;
+            pl[dim] = fc[dim] - 0.5 * fd[dim];
+            pr[dim] = fc[dim] + 0.5 * fd[dim];
+            Stencil sl = stencil_at(*sdp_, fc, pl), sr = stencil_at(*sdp_, fc, pr);
+            Combo cb;
+            double k = 1.0 / (2.0 * 0.5 * fd[dim]);
+            cb.add(sr, src, k);
+            cb.add(sl, src, -k);
+            int r = combo_row(gGrad_, cb, oU_[dim] + f);
+            (void)r;
+        }
+    }
+}
+
+}  // namespace hfh
+
+#include "hfh_flow_impl.hpp"

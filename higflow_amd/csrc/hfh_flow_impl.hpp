@@ -1,0 +1,452 @@
+// Flow<D> methods: viscoelastic recording, device layout and the step.
+#pragma once
+
+namespace hfh {
+
+// hig-flow-step-generalized-newtonian.c:14-93 (velocity gradient rows) and the
+// point evaluations of hig-flow-step-viscoelastic.c:352-544 (first tensor
+// component; the other components revisit the same cached points).
+template <int D>
+void Flow<D>::record_ve() {
+    Domain<D> &se = *sdED_;
+    int ne = se.size();
+    // velocity-derivative tensor, one combined row per (cell, i, j)
+    for (int c = 0; c < ne; c++) {
+        Vec<D> cc = se.center_of(c), cd = se.delta_of(c);
+        for (int dim = 0; dim < D; dim++)
+            for (int dim2 = 0; dim2 < D; dim2++) {
+                Domain<D> &fu = *sfdu_[dim];
+                Combo cb;
+                double k = 1.0 / (2.0 * 0.5 * cd[dim2]);
+                if (dim == dim2) {
+                    Vec<D> pl = cc, pr = cc;
+                    pl[dim] = cc[dim] - 0.5 * cd[dim];
+                    pr[dim] = cc[dim] + 0.5 * cd[dim];
+                    Stencil sl = stencil_at(fu, cc, pl), sr = stencil_at(fu, cc, pr);
+                    cb.add(sr, oU_[dim], k);
+                    cb.add(sl, oU_[dim], -k);
+                } else {
+                    // hig-flow-eval.c:160-186 then :95-121 (rr, r, l, ll)
+                    for (int side = 0; side < 2; side++) {
+                        double sg = side == 0 ? -1.0 : 1.0;
+                        Vec<D> p = cc;
+                        Stencil s4[4];
+                        p[dim] = cc[dim] + 0.5 * 1.0 * cd[dim];
+                        p[dim2] = side == 0 ? cc[dim2] - 1.0 * cd[dim2] : cc[dim2] + 1.0 * cd[dim2];
+                        s4[0] = stencil_at(fu, cc, p);
+                        p[dim] = cc[dim] + 0.5 * 1.0 * cd[dim];
+                        p[dim2] = cc[dim2];
+                        s4[1] = stencil_at(fu, cc, p);
+                        p[dim] = cc[dim] - 0.5 * 1.0 * cd[dim];
+                        p[dim2] = cc[dim2];
+                        s4[2] = stencil_at(fu, cc, p);
+                        p[dim] = cc[dim] - 0.5 * 1.0 * cd[dim];
+                        p[dim2] = side == 0 ? cc[dim2] - 1.0 * cd[dim2] : cc[dim2] + 1.0 * cd[dim2];
+                        s4[3] = stencil_at(fu, cc, p);
+                        for (int q = 0; q < 4; q++) cb.add(s4[q], oU_[dim], sg * 0.25 * k);
+                    }
+                }
+                combo_row(gDu_, cb, oDv_ + (dim * D + dim2) * ne + c);
+            }
+    }
+    // constitutive point evaluations
+    veSlots_.assign((size_t)(6 * D) * ne, oZero_placeholder());
+    veFlags_.assign(ne, 0);
+    veDelta_.assign((size_t)D * ne, 0.0);
+    for (int c = 0; c < ne; c++) {
+        Vec<D> cc = se.center_of(c), cd = se.delta_of(c);
+        for (int d = 0; d < D; d++) veDelta_[(size_t)d * ne + c] = cd[d];
+        auto S = [&](int k) -> int & { return veSlots_[(size_t)k * ne + c]; };
+        for (int dim = 0; dim < D; dim++) {  // hig_flow_velocity_at_center_cell
+            Vec<D> pl = cc, pr = cc;
+            pl[dim] = cc[dim] - 0.5 * cd[dim];
+            pr[dim] = cc[dim] + 0.5 * cd[dim];
+            S(2 * dim) = point_slot(*sfdu_[dim], oU_[dim], cc, pl, gCv_);
+            S(2 * dim + 1) = point_slot(*sfdu_[dim], oU_[dim], cc, pr, gCv_);
+        }
+        unsigned fl = 0;
+        for (int dim = 0; dim < D; dim++) {
+            const int kb = 2 * D + 4 * dim;
+            double al[4] = {1.0, 1.0, 2.0, 2.0};
+            double sg[4] = {-1.0, 1.0, -1.0, 1.0};
+            int nk = par_.ve_convec == CELL_CUBISTA ? 4 : 2;
+            for (int q = 0; q < nk; q++) {
+                Vec<D> p = cc;
+                p[dim] = sg[q] > 0 ? cc[dim] + al[q] * cd[dim] : cc[dim] - al[q] * cd[dim];
+                if (se.cell_with_point(p) >= 0) fl |= 1u << (4 * dim + q);
+                S(kb + q) = point_slot(se, oK_, cc, p, gK_, true);
+            }
+            if (par_.ve_convec == CELL_CUBISTA) {
+                Vec<D> pl = cc, pr = cc;
+                pl[dim] = cc[dim] - 0.5 * cd[dim];
+                pr[dim] = cc[dim] + 0.5 * cd[dim];
+                (void)point_slot(*sfdu_[dim], oU_[dim], cc, pl, gCv_);
+                (void)point_slot(*sfdu_[dim], oU_[dim], cc, pr, gCv_);
+            }
+        }
+        veFlags_[c] = fl;
+    }
+}
+
+template <int D>
+void Flow<D>::finalize_layout() {
+    int ne = sdED_->size();
+    gK_.ncomp = NSYM;
+    gK_.in_stride = ne;
+    oPK_ = nX_;
+    nX_ += NSYM * gK_.nrows();
+    oPcv_ = nX_;
+    nX_ += gCv_.nrows();
+    oPmom_ = nX_;
+    nX_ += gMom_.nrows();
+    oZero_ = nX_;
+    nX_ += 1;
+    for (int r = 0; r < gK_.nrows(); r++) gK_.out[r] = oPK_ + r;
+    for (int r = 0; r < gCv_.nrows(); r++) gCv_.out[r] = oPcv_ + r;
+    for (int r = 0; r < gMom_.nrows(); r++) gMom_.out[r] = oPmom_ + r;
+    auto resolve = [&](std::vector<int> &v, int base) {
+        for (auto &s : v) {
+            if (s == UNUSED) s = oZero_;
+            else if (s < 0) s = base + (-s - 1);
+        }
+    };
+    for (int d = 0; d < D; d++) resolve(momSlots_[d], oPmom_);
+    for (size_t i = 0; i < veSlots_.size(); i++) {
+        int k = (int)(i / (ne > 0 ? ne : 1));
+        int &s = veSlots_[i];
+        if (s == UNUSED) s = (k < 2 * D) ? oZero_ : oPK_;  // K slots stay valid for every component
+        else if (s < 0) s = (k < 2 * D ? oPcv_ : oPK_) + (-s - 1);
+    }
+    validate();
+    // algorithmic bytes per launch (fields read once + operator streamed)
+    auto gbytes = [&](const Group &g) {
+        return (long)g.col.size() * 12 + (long)g.nrows() * (8 + 8 + 4 + 4) * g.ncomp;
+    };
+    bytes_[ST_INTERP] = gbytes(gMom_) + gbytes(gDu_) + gbytes(gK_) + gbytes(gCv_);
+    long mb = 0;
+    for (int d = 0; d < D; d++) mb += (long)Nf_[d] * ((long)momNs_ * (4 + 8) + 4 + D * 8 + 8 + 8);
+    bytes_[ST_MOMENTUM] = mb;
+    bytes_[ST_DIV] = gbytes(gDiv_);
+    bytes_[ST_PROJECT] = gbytes(gGrad_) + (long)gGrad_.nrows() * 12 + (long)Nc_ * 24;
+    bytes_[ST_SOLVE] = (long)Aci_.size() * 12 + (long)Nc_ * 8 * 9;  // one CG iteration
+    bytes_[ST_VE] = (long)ne * (D * D * 8 * 2 + NSYM * 8 * 2 + 6 * D * (4 + 8) + 4 + D * 8);
+    if (host_only_) return;
+    // device upload
+    std::vector<double> hx(nX_, 0.0);
+    X_.alloc(nX_);
+    X_.upload(hx.data(), nX_, st_);
+    bvec_.alloc(Nc_);
+    for (int d = 0; d < D; d++) {
+        dMomSlots_[d].upload(momSlots_[d].data(), momSlots_[d].size(), st_);
+        dMomFlags_[d].upload(momFlags_[d].data(), momFlags_[d].size(), st_);
+        dMomDelta_[d].upload(momDelta_[d].data(), momDelta_[d].size(), st_);
+    }
+    if (!veSlots_.empty()) {
+        dVeSlots_.upload(veSlots_.data(), veSlots_.size(), st_);
+        dVeFlags_.upload(veFlags_.data(), veFlags_.size(), st_);
+        dVeDelta_.upload(veDelta_.data(), veDelta_.size(), st_);
+    }
+    upload_group(dDu_, gDu_, 0, 0, false);
+    upload_group(dK_, gK_, 0, gK_.nrows(), false);
+    upload_group(dCv_, gCv_, 0, 0, false);
+    upload_group(dMom_, gMom_, 0, 0, false);
+    upload_group(dDiv_, gDiv_, 0, 0, false);
+    // projection rows: u = us - dt * grad
+    {
+        std::vector<int> y(gGrad_.nrows());
+        for (int r = 0; r < gGrad_.nrows(); r++) {
+            int o = gGrad_.out[r];
+            for (int d = D - 1; d >= 0; d--)
+                if (o >= oU_[d]) {
+                    y[r] = oUS_[d] + (o - oU_[d]);
+                    break;
+                }
+        }
+        upload_group(dGrad_, gGrad_, 0, 0, true);
+        dGrad_.y.upload(y.data(), y.size(), st_);
+    }
+    psys_.set_matrix(Arp_, Aci_, Av_, st_);
+    refold_constants();
+    HFH_CHECK(hipStreamSynchronize(st_));
+}
+
+// Every index a kernel dereferences is checked on the host before upload.
+template <int D>
+void Flow<D>::validate() const {
+    auto fail = [](const char *what) { throw std::runtime_error(std::string("hfh layout check: ") + what); };
+    for (int d = 0; d < D; d++)
+        for (int s : momSlots_[d])
+            if (s < 0 || s >= nX_) fail("momentum slot out of range");
+    int ne = sdED_->size();
+    for (size_t i = 0; i < veSlots_.size(); i++) {
+        int k = (int)(i / (ne > 0 ? ne : 1));
+        int s = veSlots_[i];
+        long hi = (k < 2 * D) ? s : s + (long)(NSYM - 1) * gK_.nrows();
+        if (s < 0 || hi >= nX_) fail("constitutive slot out of range");
+    }
+    struct G {
+        const Group *g;
+        long in_hi, out_hi;
+        int ostride;
+    } gs[] = {{&gDu_, nX_, nX_, 0},
+              {&gK_, (long)nX_, (long)nX_, gK_.nrows()},
+              {&gCv_, nX_, nX_, 0},
+              {&gMom_, nX_, nX_, 0},
+              {&gDiv_, nX_, Nc_, 0},
+              {&gGrad_, nX_, nX_, 0}};
+    for (auto &e : gs) {
+        const Group &g = *e.g;
+        if ((int)g.rowptr.size() != g.nrows() + 1 || g.rowptr.back() != (int)g.col.size()) fail("row pointers");
+        for (int c : g.col)
+            if (c < 0 || c + (long)(g.ncomp - 1) * g.in_stride >= e.in_hi) fail("row column out of range");
+        for (int o : g.out)
+            if (o < 0 || o + (long)(g.ncomp - 1) * e.ostride >= e.out_hi) fail("row destination out of range");
+    }
+    for (size_t k = 0; k < Aci_.size(); k++)
+        if (Aci_[k] < 0 || Aci_[k] >= Nc_) fail("pressure matrix column out of range");
+}
+
+template <int D>
+void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool has_y) {
+    dg.nrow = g.nrows();
+    dg.ncomp = g.ncomp;
+    dg.in_stride = g.in_stride;
+    dg.out_stride = out_stride;
+    dg.has_y = has_y;
+    if (dg.nrow == 0) return;
+    dg.rp.upload(g.rowptr.data(), g.rowptr.size(), st_);
+    std::vector<int> col = g.col;
+    if (col.empty()) col.push_back(0);
+    std::vector<double> val = g.val;
+    if (val.empty()) val.push_back(0.0);
+    dg.col.upload(col.data(), col.size(), st_);
+    dg.val.upload(val.data(), val.size(), st_);
+    dg.out.upload(g.out.data(), g.out.size(), st_);
+    dg.cst.alloc(dg.nrow);
+}
+
+template <int D>
+void Flow<D>::refold_constants() {
+    struct P {
+        DGroup *d;
+        Group *g;
+    } gs[] = {{&dDu_, &gDu_}, {&dK_, &gK_}, {&dCv_, &gCv_}, {&dMom_, &gMom_}, {&dDiv_, &gDiv_}, {&dGrad_, &gGrad_}};
+    for (auto &p : gs) {
+        if (p.d->nrow == 0) continue;
+        std::vector<double> c = p.g->constants(bcval_);
+        p.d->cst.upload(c.data(), c.size(), st_);
+    }
+    HFH_CHECK(hipStreamSynchronize(st_));
+    bc_dirty_ = false;
+}
+
+template <int D>
+void Flow<D>::run_group(DGroup &dg, double alpha, double *dst, int stage) {
+    if (dg.nrow == 0) return;
+    (void)stage;
+    long n = (long)dg.nrow * dg.ncomp;
+    k_interp<double><<<nblocks(n), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, dg.val.p, dg.cst.p,
+                                                    dg.out.p, dg.has_y ? dg.y.p : nullptr, alpha, dg.in_stride,
+                                                    dg.out_stride, X_.p, dst);
+}
+
+// hig-flow-step.c:1363-1413 / hig-flow-step-viscoelastic.c:2091-2163
+template <int D>
+int Flow<D>::step(bool ve) {
+    if (!ready_ || host_only_) return -1;
+    if (par_.tempdiscrtype != EXPLICIT_EULER) return -3;
+    if (ve != (par_.flowtype == VISCOELASTIC)) return -4;
+    if (bc_dirty_) refold_constants();
+    double *X = X_.p;
+    int ne = sdED_->size();
+    if (ve) {
+        tbegin(ST_VE);
+        run_group(dDu_, 1.0, X, ST_VE);
+        run_group(dK_, 1.0, X, ST_VE);
+        run_group(dCv_, 1.0, X, ST_VE);
+        VeParams vp;
+        vp.ne = ne;
+        vp.nrowK = gK_.nrows();
+        vp.oDv = oDv_;
+        vp.oK = oK_;
+        vp.oKN = oKN_;
+        vp.oS = oS_;
+        vp.dt = par_.dt;
+        vp.Re = par_.Re;
+        vp.De = par_.De;
+        vp.beta = par_.beta;
+        vp.tol = par_.kernel_tol;
+        vp.galpha = par_.giesekus_alpha;
+        vp.model = par_.ve_model;
+        vp.convec = par_.ve_convec;
+        vp.transform = par_.kernel_transform;
+        k_constitutive<D><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
+        HFH_CHECK(hipMemcpyAsync(X + oK_, X + oKN_, sizeof(double) * NSYM * ne, hipMemcpyDeviceToDevice, st_));
+        tend(ST_VE);
+    }
+    tbegin(ST_INTERP);
+    run_group(dMom_, 1.0, X, ST_INTERP);
+    tend(ST_INTERP);
+    tbegin(ST_MOMENTUM);
+    for (int d = 0; d < D; d++) {
+        MomParams mp;
+        mp.nf = Nf_[d];
+        mp.ns = momNs_;
+        mp.u_off = oU_[d];
+        mp.us_off = oUS_[d];
+        mp.dt = par_.dt;
+        mp.Re = par_.Re;
+        mp.incremental = par_.projtype == INCREMENTAL;
+        mp.second = par_.secondconvecdiscrtype;
+        mp.ve = ve;
+        mp.dim = d;
+        if (mp.nf > 0)
+            k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[d].p, dMomFlags_[d].p, dMomDelta_[d].p,
+                                                             X);
+    }
+    tend(ST_MOMENTUM);
+    tbegin(ST_DIV);
+    run_group(dDiv_, 1.0, bvec_.p, ST_DIV);
+    tend(ST_DIV);
+    tbegin(ST_SOLVE);
+    double *x = X + (par_.projtype == INCREMENTAL ? oDP_ : oP_);
+    if (par_.precision == 1) {
+        stats_ = psys_.mixed(bvec_.p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
+    } else if (symmetric_) {
+        stats_ = psys_.pcg(bvec_.p, x, par_.rtol, par_.atol, par_.maxit, st_);
+    } else {
+        stats_ = psys_.bicgstab(bvec_.p, x, par_.rtol, par_.atol, par_.maxit, st_);
+    }
+    tend(ST_SOLVE);
+    tbegin(ST_PROJECT);
+    run_group(dGrad_, -par_.dt, X, ST_PROJECT);
+    if (par_.projtype == INCREMENTAL) k_axpy_field<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, X, oP_, oDP_);
+    tend(ST_PROJECT);
+    par_.t += par_.dt;
+    return stats_.converged ? 0 : 1;
+}
+
+// field ids as the oracle: 0 u, 1 p, 2 dp, 3 ustar, 4 F, 5 FU, 6 kernel
+// (symmetric storage, comp = i*D+j), 7 S, 8 Du
+template <int D>
+int Flow<D>::field_size(int field, int comp) {
+    if (!ready_) return -1;
+    int ne = sdED_->size();
+    switch (field) {
+    case 0:
+    case 3:
+        return comp >= 0 && comp < D ? Nf_[comp] : -1;
+    case 1:
+    case 2:
+        return Nc_;
+    case 4:
+        return sdF_->size();
+    case 5:
+        return comp >= 0 && comp < D ? sfdF_[comp]->size() : -1;
+    case 6:
+    case 7:
+    case 8:
+        return comp >= 0 && comp < D * D ? ne : -1;
+    }
+    return -1;
+}
+
+template <int D>
+int Flow<D>::field_centers(int field, int comp, double *xyz) {
+    int n = field_size(field, comp);
+    if (n < 0) return -1;
+    Domain<D> *dm = field == 0 || field == 3 ? sfdu_[comp].get()
+                    : field == 5            ? sfdF_[comp].get()
+                    : field == 4            ? sdF_.get()
+                    : field >= 6            ? sdED_.get()
+                                            : sdp_.get();
+    for (int i = 0; i < n; i++) {
+        Vec<D> c = dm->center_of(i);
+        for (int d = 0; d < D; d++) xyz[(size_t)i * D + d] = c[d];
+    }
+    return n;
+}
+
+static inline int sym_index(int D, int i, int j) {
+    if (i > j) std::swap(i, j);
+    int m = 0;
+    for (int a = 0; a < D; a++)
+        for (int b = a; b < D; b++, m++)
+            if (a == i && b == j) return m;
+    return -1;
+}
+
+template <int D>
+int Flow<D>::get_field(int field, int comp, double *out) {
+    int n = field_size(field, comp);
+    if (n < 0 || host_only_) return -1;
+    int off;
+    int ne = sdED_->size();
+    switch (field) {
+    case 0: off = oU_[comp]; break;
+    case 1: off = oP_; break;
+    case 2: off = oDP_; break;
+    case 3: off = oUS_[comp]; break;
+    case 4: off = oF_; break;
+    case 5: off = oFU_[comp]; break;
+    case 6: off = oK_ + sym_index(D, comp / D, comp % D) * ne; break;
+    case 7: off = oS_ + comp * ne; break;
+    default: off = oDv_ + comp * ne; break;
+    }
+    HFH_CHECK(hipMemcpyAsync(out, X_.p + off, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
+    HFH_CHECK(hipStreamSynchronize(st_));
+    return n;
+}
+
+template <int D>
+int Flow<D>::set_field(int field, int comp, const double *in) {
+    int n = field_size(field, comp);
+    if (n < 0 || host_only_) return -1;
+    int off;
+    int ne = sdED_->size();
+    switch (field) {
+    case 0: off = oU_[comp]; break;
+    case 1: off = oP_; break;
+    case 2: off = oDP_; break;
+    case 3: off = oUS_[comp]; break;
+    case 4: off = oF_; break;
+    case 5: off = oFU_[comp]; break;
+    case 6:
+        if (comp / D > comp % D) return n;  // symmetric storage keeps the upper triangle
+        off = oK_ + sym_index(D, comp / D, comp % D) * ne;
+        break;
+    case 7: off = oS_ + comp * ne; break;
+    default: off = oDv_ + comp * ne; break;
+    }
+    HFH_CHECK(hipMemcpyAsync(X_.p + off, in, sizeof(double) * n, hipMemcpyHostToDevice, st_));
+    HFH_CHECK(hipStreamSynchronize(st_));
+    return n;
+}
+
+template <int D>
+int Flow<D>::stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn, int *ids,
+                     double *w, double *rhs) {
+    if (!ready_) return -1;
+    Domain<D> *dm = domain == 0 ? sdp_.get() : domain == 2 ? sdED_.get() : nullptr;
+    if (domain == 1) {
+        if (dim < 0 || dim >= D) return -1;
+        dm = sfdu_[dim].get();
+    }
+    if (!dm) return -1;
+    Vec<D> c, p;
+    for (int d = 0; d < D; d++) {
+        c[d] = center[d];
+        p[d] = x[d];
+    }
+    Stencil s;
+    dm->get_stencil(c, p, alpha, s);
+    int n = (int)s.ids.size();
+    for (int i = 0; i < n && i < maxn; i++) {
+        ids[i] = s.ids[i];
+        w[i] = s.w[i];
+    }
+    *rhs = s.rhs(bcval_);
+    return n;
+}
+
+}  // namespace hfh

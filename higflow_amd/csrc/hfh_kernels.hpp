@@ -1,0 +1,216 @@
+// Device kernels of the projection step (gfx950, wave64).
+//
+// All FP64 arithmetic mirrors the reference's operation order
+// (hig-flow-discret.c, hig-flow-step.c, hig-flow-step-viscoelastic.c) and the
+// library is built with -ffp-contract=off, so a kernel and the CPU reference
+// round identically whenever they see identical inputs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hfh {
+
+constexpr int BLOCK = 256;
+
+#define HFH_FLT_EQ(x, y) (fabs((x) - (y)) < 1.0e-12)
+
+// ---------------------------------------------------------------- interp
+// Affine rows over the state vector X:
+//   v = cst[r] + sum_k val[k] * X[col[k] + c*in_stride]
+//   dst[out[r] + c*out_stride] = alpha * v + (yoff ? X[yoff[r]] : 0)
+// One thread per (row, component); rows are short (identity/ghost rows) or
+// WLS rows of up to 45/120 entries kept contiguous.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_interp(int nrow, int ncomp, const int *__restrict__ rowptr,
+                                                  const int *__restrict__ col, const T *__restrict__ val,
+                                                  const T *__restrict__ cst, const int *__restrict__ out,
+                                                  const int *__restrict__ yoff, T alpha, int in_stride,
+                                                  int out_stride, const T *X, T *dst) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrow * ncomp) return;
+    int r = t % nrow, c = t / nrow;
+    int b = rowptr[r], e = rowptr[r + 1];
+    T v = cst[r];
+    const T *Xc = X + (size_t)c * in_stride;
+    for (int k = b; k < e; k++) v += Xc[col[k]] * val[k];
+    T o = alpha * v;
+    if (yoff) o = X[yoff[r]] + o;
+    dst[out[r] + (size_t)c * out_stride] = o;
+}
+
+// ------------------------------------------------------------- momentum
+struct MomParams {
+    int nf, ns;
+    int u_off, us_off;
+    double dt, Re;
+    int incremental, second, ve, dim;
+    double delta[3];  // unused when per-facet deltas are given
+};
+
+// Per-facet slot layout (produced by the recorder):
+//   0 F, 1 dpdx, 2+6*d2+{0..5}: ul, ur, ull, urr, vcR, vcL (neighbours in
+//   direction d2 and the d2-velocity at fc -/+ delta/2), then D-1 transverse
+//   velocities, then D tensor divergences (viscoelastic).
+// flags: bits 0-1 final convective type, bit 2+d2 infacet_rr, bit 2+D+d2
+// infacet_ll (hig-flow-discret.c:131-251).
+template <int D>
+__global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__restrict__ slots,
+                                                    const unsigned *__restrict__ flags,
+                                                    const double *__restrict__ fdelta, double *X) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= P.nf) return;
+    const int nf = P.nf;
+    const int dim = P.dim;
+    const double a = 1.7500, b = 0.3750, c = 0.7500, d = 0.1250, e = 0.2500;
+    unsigned fl = flags[f];
+    int ctype = fl & 3;
+    double fd[D];
+    for (int k = 0; k < D; k++) fd[k] = fdelta[(size_t)k * nf + f];
+    double uf = X[P.u_off + f];
+    double F = X[slots[f]];
+    double dpdx = X[slots[(size_t)nf + f]];
+    double conv = 0.0, diff = 0.0, tens = 0.0;
+    int vk = 2 + 6 * D;
+    for (int d2 = 0; d2 < D; d2++) {
+        const int base = 2 + 6 * d2;
+        double ul = X[slots[(size_t)(base + 0) * nf + f]];
+        double ur = X[slots[(size_t)(base + 1) * nf + f]];
+        double v;
+        if (d2 == dim) {
+            v = uf;
+        } else {
+            v = X[slots[(size_t)vk * nf + f]];
+            vk++;
+        }
+        double dudxc = (ur - ul) / (2.0 * 1.0 * fd[d2]);
+        double dudxr = (ur - uf) / (1.0 * fd[d2]);
+        double dudxl = (uf - ul) / (1.0 * fd[d2]);
+        if (ctype == 2) {
+            if (P.second == 1) {
+                double ull = X[slots[(size_t)(base + 3) * nf + f]];
+                double urr = X[slots[(size_t)(base + 2) * nf + f]];
+                int irr = (fl >> (2 + d2)) & 1, ill = (fl >> (2 + D + d2)) & 1;
+                double vc = X[slots[(size_t)(base + 4) * nf + f]];
+                double conv1 = 0.0, conv2 = 0.0, fi;
+                if (vc > 0.0) {
+                    if (HFH_FLT_EQ(ur, ul)) {
+                        conv1 = vc * uf;
+                    } else {
+                        fi = (uf - ul) / (ur - ul);
+                        if ((fi <= 0.0) || (fi >= 1.0)) {
+                            conv1 = vc * uf;
+                        } else {
+                            if (fi < b) conv1 = vc * (a * uf - c * ul);
+                            if ((fi >= b) && (fi <= c)) conv1 = vc * (c * uf + b * ur - d * ul);
+                            if (fi > c) conv1 = vc * (e * uf + c * ur);
+                        }
+                    }
+                } else {
+                    if (HFH_FLT_EQ(uf, urr)) {
+                        conv1 = vc * ur;
+                    } else {
+                        fi = (ur - urr) / (uf - urr);
+                        if ((fi <= 0.0) || (fi >= 1.0)) {
+                            conv1 = vc * ur;
+                        } else {
+                            if (fi < b) conv1 = irr ? vc * (a * ur - c * urr) : vc * ur;
+                            if ((fi >= b) && (fi <= c)) conv1 = irr ? vc * (c * ur + b * uf - d * urr) : vc * ur;
+                            if (fi > c) conv1 = vc * (c * uf + e * ur);
+                        }
+                    }
+                }
+                vc = X[slots[(size_t)(base + 5) * nf + f]];
+                if (vc > 0.0) {
+                    if (HFH_FLT_EQ(uf, ull)) {
+                        conv2 = vc * ul;
+                    } else {
+                        fi = (ul - ull) / (uf - ull);
+                        if ((fi <= 0.0) || (fi >= 1.0)) {
+                            conv2 = vc * ul;
+                        } else {
+                            if (fi < b) conv2 = ill ? vc * (a * ul - c * ull) : vc * ul;
+                            if ((fi >= b) && (fi <= c)) conv2 = ill ? vc * (b * uf + c * ul - d * ull) : vc * ul;
+                            if (fi > c) conv2 = vc * (c * uf + e * ul);
+                        }
+                    }
+                } else {
+                    if (HFH_FLT_EQ(ul, ur)) {
+                        conv2 = vc * uf;
+                    } else {
+                        fi = (uf - ur) / (ul - ur);
+                        if ((fi <= 0.0) || (fi >= 1.0)) {
+                            conv2 = vc * uf;
+                        } else {
+                            if (fi < b) conv2 = vc * (a * uf - c * ur);
+                            if ((fi >= b) && (fi <= c)) conv2 = vc * (b * ul + c * uf - d * ur);
+                            if (fi > c) conv2 = vc * (c * ul + e * uf);
+                        }
+                    }
+                }
+                conv += ((conv1 - conv2) / fd[dim]);
+            } else if (P.second == 2) {
+                double urr = X[slots[(size_t)(base + 2) * nf + f]];
+                double ull = X[slots[(size_t)(base + 3) * nf + f]];
+                double dudxrr = (urr - ur) / (1.0 * fd[d2]);
+                double dudxll = (uf - ull) / (1.0 * fd[d2]);
+                (void)dudxll;
+                double dll = (ul - ull) / (1.0 * fd[d2]);
+                conv += (v + fabs(v)) * 0.1875 * dudxl;
+                conv += (v + fabs(v)) * 0.3750 * dudxr;
+                conv -= (v + fabs(v)) * 0.0625 * dudxrr;
+                conv += (v - fabs(v)) * 0.3750 * dudxl;
+                conv += (v - fabs(v)) * 0.1875 * dudxr;
+                conv -= (v - fabs(v)) * 0.0625 * dll;
+            } else {
+                double du2 = (ul - 2.0 * uf + ur) / (fd[d2] * fd[d2] * 1.0 * 1.0);
+                conv += 0.5 * (v + fabs(v)) * dudxl;
+                conv += 0.5 * (v - fabs(v)) * dudxr;
+                if (v > 0.0) {
+                    if (fabs(dudxl) > 1.0e-8) conv += 0.5 * v * fd[d2] * du2 / dudxl;
+                } else {
+                    if (fabs(dudxr) > 1.0e-8) conv -= 0.5 * v * fd[d2] * du2 / dudxr;
+                }
+            }
+        } else if (ctype == 1) {
+            conv += 0.5 * (v + fabs(v)) * dudxl;
+            conv += 0.5 * (v - fabs(v)) * dudxr;
+        } else {
+            conv += v * dudxc;
+        }
+        diff += (ul - 2.0 * uf + ur) / (fd[d2] * fd[d2] * 1.0 * 1.0);
+    }
+    diff /= P.Re;
+    double rhs = 0.0;
+    double pt = P.incremental ? dpdx : 0.0;
+    if (!P.ve) {
+        rhs -= pt;
+        rhs -= conv;
+        rhs += diff;
+        rhs += F;
+    } else {
+        int tk = 2 + 6 * D + (D - 1);
+        for (int d2 = 0; d2 < D; d2++) tens += X[slots[(size_t)(tk + d2) * nf + f]];
+        rhs += F;
+        rhs -= pt;
+        rhs += tens;
+        rhs -= conv;
+        rhs += diff;
+    }
+    X[P.us_off + f] = uf + P.dt * rhs;
+}
+
+// p += dp over all cells (hig-flow-step.c:311-340)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_axpy_field(int n, T *X, int dst, int src) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) X[dst + i] = X[dst + i] + X[src + i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_copy(int n, const T *src, T *dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace hfh

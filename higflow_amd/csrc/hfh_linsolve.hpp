@@ -1,0 +1,558 @@
+// Device Krylov solvers for the pressure-Poisson and solver-plugin systems.
+//
+// CSR operator resident in HBM, all scalars resident on the device (no host
+// round trip inside an iteration); the host polls the residual every
+// `check_every` iterations.  Dot products are two-stage (per-workgroup
+// partials, then one workgroup) and therefore deterministic.
+//
+//   pcg()       Jacobi-preconditioned CG for SPD systems (the pinned
+//               pressure system after symmetric elimination).
+//   bicgstab()  right Jacobi-preconditioned BiCGStab for nonsymmetric systems
+//               (the solver-plugin default, as -ksp_type bcgs of the reference
+//               run lines, higtree/src/solver-petsc.c:387).
+//   mixed()     FP64 defect correction around an FP32 inner PCG.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hfh_kernels.hpp"
+
+#define HFH_CHECK(x)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) +    \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__));   \
+    } while (0)
+
+namespace hfh {
+
+// ------------------------------------------------------------ reductions
+template <typename T>
+__device__ inline T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <typename T>
+__device__ inline T block_sum(T v) {
+    __shared__ T sh[BLOCK / 64];
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    T r = 0;
+    if (w == 0) {
+        r = lane < BLOCK / 64 ? sh[lane] : T(0);
+        r = wave_sum(r);
+    }
+    __syncthreads();
+    return r;
+}
+
+// scalar slots
+enum { S_RHO = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_RHONEW, S_OMEGA, S_TS, S_TT, S_RV, S_BB, S_NSC };
+
+// q = A p, partial p.q
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_spmv_dot(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                    const T *__restrict__ v, const T *__restrict__ p, T *__restrict__ q,
+                                                    T *__restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T acc = 0;
+    if (i < n) {
+        T s = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * p[ci[k]];
+        q[i] = s;
+        acc = p[i] * s;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// y = A x (no reduction)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_spmv(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        T s = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * x[ci[k]];
+        y[i] = s;
+    }
+}
+
+// sums nb partials of up to 2 arrays into sc[o1], sc[o2]
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_reduce(int nb, const T *__restrict__ p1, const T *__restrict__ p2,
+                                                  T *sc, int o1, int o2) {
+    T a = 0, b = 0;
+    for (int i = threadIdx.x; i < nb; i += BLOCK) {
+        a += p1[i];
+        if (p2) b += p2[i];
+    }
+    a = block_sum(a);
+    b = block_sum(b);
+    if (threadIdx.x == 0) {
+        sc[o1] = a;
+        if (p2) sc[o2] = b;
+    }
+}
+
+// CG: alpha = rho / pq
+template <typename T>
+__global__ void k_cg_alpha(T *sc) {
+    if (threadIdx.x == 0) {
+        T pq = sc[S_PQ];
+        sc[S_ALPHA] = pq != T(0) ? sc[S_RHO] / pq : T(0);
+    }
+}
+
+// x += a p; r -= a q; z = dinv r; partials r.z and r.r
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_cg_update(int n, T *__restrict__ x, T *__restrict__ r,
+                                                     const T *__restrict__ p, const T *__restrict__ q,
+                                                     const T *__restrict__ dinv, T *__restrict__ z,
+                                                     const T *__restrict__ sc, T *__restrict__ prz,
+                                                     T *__restrict__ prr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T rz = 0, rr = 0;
+    if (i < n) {
+        T a = sc[S_ALPHA];
+        x[i] = x[i] + a * p[i];
+        T ri = r[i] - a * q[i];
+        r[i] = ri;
+        T zi = dinv[i] * ri;
+        z[i] = zi;
+        rz = ri * zi;
+        rr = ri * ri;
+    }
+    rz = block_sum(rz);
+    rr = block_sum(rr);
+    if (threadIdx.x == 0) {
+        prz[blockIdx.x] = rz;
+        prr[blockIdx.x] = rr;
+    }
+}
+
+template <typename T>
+__global__ void k_cg_beta(T *sc) {
+    if (threadIdx.x == 0) {
+        T rho = sc[S_RHO], rn = sc[S_RHONEW];
+        sc[S_BETA] = rho != T(0) ? rn / rho : T(0);
+        sc[S_RHO] = rn;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_cg_p(int n, T *__restrict__ p, const T *__restrict__ z,
+                                                const T *__restrict__ sc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + sc[S_BETA] * p[i];
+}
+
+// r = b - A x, z = dinv r, p = z, partial r.z, r.r, b.b
+template <typename T, typename TB>
+__global__ void __launch_bounds__(BLOCK) k_cg_init(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                   const T *__restrict__ v, const T *__restrict__ x,
+                                                   const TB *__restrict__ b, const T *__restrict__ dinv,
+                                                   T *__restrict__ r, T *__restrict__ z, T *__restrict__ p,
+                                                   T *__restrict__ prz, T *__restrict__ prr, T *__restrict__ pbb) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T rz = 0, rr = 0, bb = 0;
+    if (i < n) {
+        T s = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * x[ci[k]];
+        T bi = (T)b[i];
+        T ri = bi - s;
+        r[i] = ri;
+        T zi = dinv[i] * ri;
+        z[i] = zi;
+        p[i] = zi;
+        rz = ri * zi;
+        rr = ri * ri;
+        bb = bi * bi;
+    }
+    rz = block_sum(rz);
+    rr = block_sum(rr);
+    bb = block_sum(bb);
+    if (threadIdx.x == 0) {
+        prz[blockIdx.x] = rz;
+        prr[blockIdx.x] = rr;
+        pbb[blockIdx.x] = bb;
+    }
+}
+
+// generic helpers for BiCGStab
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_dot2(int n, const T *__restrict__ a, const T *__restrict__ b,
+                                                const T *__restrict__ c, const T *__restrict__ d,
+                                                T *__restrict__ p1, T *__restrict__ p2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T s1 = 0, s2 = 0;
+    if (i < n) {
+        s1 = a[i] * b[i];
+        if (c) s2 = c[i] * d[i];
+    }
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        p1[blockIdx.x] = s1;
+        if (p2) p2[blockIdx.x] = s2;
+    }
+}
+
+// BiCGStab p = r + beta (p - omega v); ph = dinv p
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_bcg_p(int n, int first, T *__restrict__ p, const T *__restrict__ r,
+                                                 const T *__restrict__ vv, const T *__restrict__ dinv,
+                                                 T *__restrict__ ph, const T *__restrict__ sc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        T pi;
+        if (first) {
+            pi = r[i];
+        } else {
+            T beta = (sc[S_RHONEW] / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]);
+            pi = r[i] + beta * (p[i] - sc[S_OMEGA] * vv[i]);
+        }
+        p[i] = pi;
+        ph[i] = dinv[i] * pi;
+    }
+}
+
+// s = r - alpha v (alpha = rhonew / rv); sh = dinv s; partial s.s
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_bcg_s(int n, const T *__restrict__ r, const T *__restrict__ vv,
+                                                 const T *__restrict__ dinv, T *__restrict__ s, T *__restrict__ sh,
+                                                 T *sc, T *__restrict__ pss) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T alpha = sc[S_RV] != T(0) ? sc[S_RHONEW] / sc[S_RV] : T(0);
+    T ss = 0;
+    if (i < n) {
+        T si = r[i] - alpha * vv[i];
+        s[i] = si;
+        sh[i] = dinv[i] * si;
+        ss = si * si;
+    }
+    ss = block_sum(ss);
+    if (threadIdx.x == 0) pss[blockIdx.x] = ss;
+}
+
+template <typename T>
+__global__ void k_bcg_alpha(T *sc) {
+    if (threadIdx.x == 0) sc[S_ALPHA] = sc[S_RV] != T(0) ? sc[S_RHONEW] / sc[S_RV] : T(0);
+}
+
+// x += alpha ph + omega sh; r = s - omega t; partial r.r, rh.r
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_bcg_x(int n, T *__restrict__ x, T *__restrict__ r,
+                                                 const T *__restrict__ ph, const T *__restrict__ sh,
+                                                 const T *__restrict__ s, const T *__restrict__ t,
+                                                 const T *__restrict__ rh, T *sc, T *__restrict__ prr,
+                                                 T *__restrict__ prhr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T omega = sc[S_TT] > T(0) ? sc[S_TS] / sc[S_TT] : T(0);
+    T rr = 0, rhr = 0;
+    if (i < n) {
+        x[i] = x[i] + sc[S_ALPHA] * ph[i] + omega * sh[i];
+        T ri = s[i] - omega * t[i];
+        r[i] = ri;
+        rr = ri * ri;
+        rhr = rh[i] * ri;
+    }
+    rr = block_sum(rr);
+    rhr = block_sum(rhr);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = rr;
+        prhr[blockIdx.x] = rhr;
+    }
+}
+
+template <typename T>
+__global__ void k_bcg_shift(T *sc) {
+    if (threadIdx.x == 0) {
+        sc[S_OMEGA] = sc[S_TT] > T(0) ? sc[S_TS] / sc[S_TT] : T(0);
+        sc[S_RHO] = sc[S_RHONEW];
+    }
+}
+
+// x += alpha ph (early exit when |s| converged)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_axpy_sc(int n, T *x, const T *y, const T *sc, int slot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + sc[slot] * y[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_fill(int n, T *x, T v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+template <typename TD, typename TS>
+__global__ void __launch_bounds__(BLOCK) k_convert(int n, const TS *a, TD *b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (TD)a[i];
+}
+
+template <typename TD, typename TS>
+__global__ void __launch_bounds__(BLOCK) k_add_conv(int n, TD *x, const TS *e) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + (TD)e[i];
+}
+
+// r = b - A x (FP64), partial r.r and b.b
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_residual(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                    const T *__restrict__ v, const T *__restrict__ x,
+                                                    const T *__restrict__ b, T *__restrict__ r, T *__restrict__ prr,
+                                                    T *__restrict__ pbb) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    T rr = 0, bb = 0;
+    if (i < n) {
+        T s = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * x[ci[k]];
+        T ri = b[i] - s;
+        r[i] = ri;
+        rr = ri * ri;
+        bb = b[i] * b[i];
+    }
+    rr = block_sum(rr);
+    bb = block_sum(bb);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = rr;
+        pbb[blockIdx.x] = bb;
+    }
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t k) {
+        free();
+        if (k == 0) k = 1;
+        HFH_CHECK(hipMalloc((void **)&p, k * sizeof(T)));
+        n = k;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t k, hipStream_t s) {
+        if (k > n) alloc(k);
+        if (k) HFH_CHECK(hipMemcpyAsync(p, h, k * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    ~DevBuf() { free(); }
+};
+
+struct SolveStats {
+    int iters = 0;
+    double resid = 0.0;  // ||b - Ax|| / ||b||
+    int converged = 0;
+    int outer = 0;       // refinement steps (mixed)
+};
+
+// A sparse system on one device.
+class DeviceSystem {
+  public:
+    int n = 0, nnz = 0;
+    DevBuf<int> rp, ci;
+    DevBuf<double> v, dinv;
+    DevBuf<float> v32, dinv32;
+    DevBuf<double> r, z, p, q, rh, s, sh, t, ph, bb_tmp;
+    DevBuf<float> r32, z32, p32, q32, x32, b32;
+    DevBuf<double> part1, part2, part3, sc;
+    DevBuf<float> fpart1, fpart2, fpart3, fsc;
+    int nb = 0;
+    int check_every = 8;
+
+    void set_matrix(const std::vector<int> &hrp, const std::vector<int> &hci, const std::vector<double> &hv,
+                    hipStream_t st) {
+        n = (int)hrp.size() - 1;
+        nnz = (int)hci.size();
+        rp.upload(hrp.data(), hrp.size(), st);
+        ci.upload(hci.data(), hci.size(), st);
+        v.upload(hv.data(), hv.size(), st);
+        std::vector<double> hd(n, 1.0);
+        std::vector<float> hv32(hv.begin(), hv.end());
+        for (int i = 0; i < n; i++)
+            for (int k = hrp[i]; k < hrp[i + 1]; k++)
+                if (hci[k] == i && hv[k] != 0.0) hd[i] = 1.0 / hv[k];
+        std::vector<float> hd32(hd.begin(), hd.end());
+        dinv.upload(hd.data(), n, st);
+        v32.upload(hv32.data(), hv32.size(), st);
+        dinv32.upload(hd32.data(), n, st);
+        nb = nblocks(n);
+        for (auto *b : {&r, &z, &p, &q, &rh, &s, &sh, &t, &ph, &bb_tmp})
+            if ((int)b->n < n) b->alloc(n);
+        for (auto *b : {&r32, &z32, &p32, &q32, &x32, &b32})
+            if ((int)b->n < n) b->alloc(n);
+        for (auto *b : {&part1, &part2, &part3})
+            if ((int)b->n < nb) b->alloc(nb);
+        for (auto *b : {&fpart1, &fpart2, &fpart3})
+            if ((int)b->n < nb) b->alloc(nb);
+        if (!sc.p) sc.alloc(S_NSC);
+        if (!fsc.p) fsc.alloc(S_NSC);
+        HFH_CHECK(hipStreamSynchronize(st));
+    }
+
+    // Jacobi-PCG, FP64.  x holds the initial guess (warm start).
+    SolveStats pcg(const double *b, double *x, double rtol, double atol, int maxit, hipStream_t st) {
+        SolveStats S;
+        k_cg_init<double, double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, dinv.p, r.p, z.p, p.p, part1.p,
+                                                         part2.p, part3.p);
+        k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RHO, S_RR);
+        k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part3.p, nullptr, sc.p, S_BB, 0);
+        double h[S_NSC];
+        HFH_CHECK(hipMemcpyAsync(h, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipStreamSynchronize(st));
+        double bn = std::sqrt(h[S_BB]);
+        double tol = std::max(rtol * bn, atol);
+        double rn = std::sqrt(h[S_RR]);
+        int it = 0;
+        while (rn > tol && it < maxit) {
+            int burst = std::min(check_every, maxit - it);
+            for (int k = 0; k < burst; k++) {
+                k_spmv_dot<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, p.p, q.p, part1.p);
+                k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, nullptr, sc.p, S_PQ, 0);
+                k_cg_alpha<double><<<1, 64, 0, st>>>(sc.p);
+                k_cg_update<double><<<nb, BLOCK, 0, st>>>(n, x, r.p, p.p, q.p, dinv.p, z.p, sc.p, part1.p,
+                                                          part2.p);
+                k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RHONEW, S_RR);
+                k_cg_beta<double><<<1, 64, 0, st>>>(sc.p);
+                k_cg_p<double><<<nb, BLOCK, 0, st>>>(n, p.p, z.p, sc.p);
+            }
+            it += burst;
+            HFH_CHECK(hipMemcpyAsync(h, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+            rn = std::sqrt(h[S_RR]);
+            if (!std::isfinite(rn)) break;
+        }
+        S.iters = it;
+        S.resid = bn > 0 ? rn / bn : rn;
+        S.converged = rn <= tol;
+        return S;
+    }
+
+    // FP32 Jacobi-PCG on (A32) e = r, e starts at 0; relative tolerance only.
+    int pcg32(double inner_rtol, int maxit, hipStream_t st) {
+        k_fill<float><<<nb, BLOCK, 0, st>>>(n, x32.p, 0.0f);
+        k_cg_init<float, float><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v32.p, x32.p, b32.p, dinv32.p, r32.p, z32.p,
+                                                       p32.p, fpart1.p, fpart2.p, fpart3.p);
+        k_reduce<float><<<1, BLOCK, 0, st>>>(nb, fpart1.p, fpart2.p, fsc.p, S_RHO, S_RR);
+        float h[S_NSC];
+        HFH_CHECK(hipMemcpyAsync(h, fsc.p, sizeof h, hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipStreamSynchronize(st));
+        double r0 = std::sqrt((double)h[S_RR]);
+        double tol = inner_rtol * r0;
+        double rn = r0;
+        int it = 0;
+        while (rn > tol && it < maxit) {
+            int burst = std::min(check_every, maxit - it);
+            for (int k = 0; k < burst; k++) {
+                k_spmv_dot<float><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v32.p, p32.p, q32.p, fpart1.p);
+                k_reduce<float><<<1, BLOCK, 0, st>>>(nb, fpart1.p, nullptr, fsc.p, S_PQ, 0);
+                k_cg_alpha<float><<<1, 64, 0, st>>>(fsc.p);
+                k_cg_update<float><<<nb, BLOCK, 0, st>>>(n, x32.p, r32.p, p32.p, q32.p, dinv32.p, z32.p, fsc.p,
+                                                         fpart1.p, fpart2.p);
+                k_reduce<float><<<1, BLOCK, 0, st>>>(nb, fpart1.p, fpart2.p, fsc.p, S_RHONEW, S_RR);
+                k_cg_beta<float><<<1, 64, 0, st>>>(fsc.p);
+                k_cg_p<float><<<nb, BLOCK, 0, st>>>(n, p32.p, z32.p, fsc.p);
+            }
+            it += burst;
+            HFH_CHECK(hipMemcpyAsync(h, fsc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+            rn = std::sqrt((double)h[S_RR]);
+            if (!std::isfinite(rn)) break;
+        }
+        return it;
+    }
+
+    // FP64 iterative refinement around the FP32 PCG.
+    SolveStats mixed(const double *b, double *x, double rtol, double atol, double inner_rtol, int maxit,
+                     hipStream_t st) {
+        SolveStats S;
+        double h[S_NSC];
+        int outer = 0, inner_total = 0;
+        double bn = 0, rn = 0, tol = 0;
+        for (;;) {
+            k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
+            HFH_CHECK(hipMemcpyAsync(h, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+            bn = std::sqrt(h[S_BB]);
+            rn = std::sqrt(h[S_RR]);
+            tol = std::max(rtol * bn, atol);
+            if (rn <= tol || outer >= 50 || inner_total >= maxit || !std::isfinite(rn)) break;
+            k_convert<float, double><<<nb, BLOCK, 0, st>>>(n, r.p, b32.p);
+            inner_total += pcg32(inner_rtol, maxit - inner_total, st);
+            k_add_conv<double, float><<<nb, BLOCK, 0, st>>>(n, x, x32.p);
+            outer++;
+        }
+        S.iters = inner_total;
+        S.outer = outer;
+        S.resid = bn > 0 ? rn / bn : rn;
+        S.converged = rn <= tol;
+        return S;
+    }
+
+    // Right Jacobi-preconditioned BiCGStab, FP64 (van der Vorst 1992).
+    SolveStats bicgstab(const double *b, double *x, double rtol, double atol, int maxit, hipStream_t st) {
+        SolveStats S;
+        double h[S_NSC];
+        k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+        k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
+        HFH_CHECK(hipMemcpyAsync(rh.p, r.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HFH_CHECK(hipMemcpyAsync(h, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipStreamSynchronize(st));
+        double bn = std::sqrt(h[S_BB]);
+        double tol = std::max(rtol * bn, atol);
+        double rn = std::sqrt(h[S_RR]);
+        // rho(rh, r) = rr at start
+        double init[S_NSC] = {0};
+        init[S_RHONEW] = h[S_RR];
+        init[S_RHO] = 1.0;
+        init[S_ALPHA] = 1.0;
+        init[S_OMEGA] = 1.0;
+        HFH_CHECK(hipMemcpyAsync(sc.p, init, sizeof init, hipMemcpyHostToDevice, st));
+        int it = 0;
+        while (rn > tol && it < maxit) {
+            k_bcg_p<double><<<nb, BLOCK, 0, st>>>(n, it == 0, p.p, r.p, q.p, dinv.p, ph.p, sc.p);
+            k_spmv<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, ph.p, q.p);  // v = A ph (q holds v)
+            k_dot2<double><<<nb, BLOCK, 0, st>>>(n, rh.p, q.p, nullptr, nullptr, part1.p, nullptr);
+            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, nullptr, sc.p, S_RV, 0);
+            k_bcg_alpha<double><<<1, 64, 0, st>>>(sc.p);
+            k_bcg_s<double><<<nb, BLOCK, 0, st>>>(n, r.p, q.p, dinv.p, s.p, sh.p, sc.p, part1.p);
+            k_spmv<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, sh.p, t.p);
+            k_dot2<double><<<nb, BLOCK, 0, st>>>(n, t.p, s.p, t.p, t.p, part2.p, part3.p);
+            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part2.p, part3.p, sc.p, S_TS, S_TT);
+            k_bcg_x<double><<<nb, BLOCK, 0, st>>>(n, x, r.p, ph.p, sh.p, s.p, t.p, rh.p, sc.p, part1.p, part2.p);
+            k_bcg_shift<double><<<1, 64, 0, st>>>(sc.p);
+            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_RHONEW);
+            it++;
+            if (it % check_every == 0 || it == maxit) {
+                HFH_CHECK(hipMemcpyAsync(h, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+                HFH_CHECK(hipStreamSynchronize(st));
+                rn = std::sqrt(h[S_RR]);
+                if (!std::isfinite(rn) || h[S_RHONEW] == 0.0) break;
+            }
+        }
+        HFH_CHECK(hipMemcpyAsync(h, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipStreamSynchronize(st));
+        rn = std::sqrt(h[S_RR]);
+        S.iters = it;
+        S.resid = bn > 0 ? rn / bn : rn;
+        S.converged = rn <= tol;
+        return S;
+    }
+};
+
+}  // namespace hfh

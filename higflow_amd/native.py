@@ -1,0 +1,289 @@
+"""Python host mirror of the HiG-Flow projection step and solver-plugin API,
+bound to ``libhigflow_hip.so`` (the HIP/gfx950 library; C ABI in
+``include/higflow_hip.h``).
+
+There is no CPU fallback: every call goes through the HIP library, and
+importing the binding raises when the library is missing or no GPU is
+visible for a call that needs one.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libhigflow_hip.so")
+
+STAGES = ("interp", "momentum", "divergence", "solve", "project", "viscoelastic")
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """hfh_params (include/higflow_hip.h)."""
+    _fields_ = [
+        ("Re", C.c_double), ("dt", C.c_double), ("t", C.c_double),
+        ("projtype", C.c_int), ("flowtype", C.c_int), ("tempdiscrtype", C.c_int),
+        ("convecdiscrtype", C.c_int), ("secondconvecdiscrtype", C.c_int),
+        ("ve_model", C.c_int), ("ve_convec", C.c_int), ("kernel_transform", C.c_int),
+        ("De", C.c_double), ("beta", C.c_double), ("kernel_tol", C.c_double),
+        ("giesekus_alpha", C.c_double),
+        ("rtol", C.c_double), ("atol", C.c_double), ("maxit", C.c_int),
+        ("pin_pressure", C.c_double), ("order", C.c_int),
+        ("precision", C.c_int), ("inner_rtol", C.c_double),
+    ]
+
+
+_LIB = None
+
+
+def load():
+    """Load libhigflow_hip.so (raises HipError when it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise HipError("libhigflow_hip.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIBPATH)
+    vp, dp, ip, cp, lp = (C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_char_p,
+                          C.POINTER(C.c_long))
+    L.hfh_last_error.restype = C.c_char_p
+    L.hfh_slv_create.restype = vp
+    L.hfh_slv_create.argtypes = [C.c_int, C.c_int]
+    L.hfh_slv_set_maxnonzeros.argtypes = [vp, C.c_int]
+    L.hfh_slv_set_Ai.argtypes = [vp, C.c_int, C.c_int, ip, dp]
+    for f in ("hfh_slv_set_bi", "hfh_slv_add_bi", "hfh_slv_set_xi", "hfh_slv_impose_value"):
+        getattr(L, f).argtypes = [vp, C.c_int, C.c_double]
+    L.hfh_slv_set_x.argtypes = [vp, dp]
+    L.hfh_slv_get_x.argtypes = [vp, dp]
+    L.hfh_slv_get_xi.argtypes = [vp, C.c_int]
+    L.hfh_slv_get_xi.restype = C.c_double
+    L.hfh_slv_get_x_scatter.argtypes = [vp, C.c_int, ip, dp]
+    for f in ("hfh_slv_assemble_matrix", "hfh_slv_assemble_rhs", "hfh_slv_assemble_output",
+              "hfh_slv_solve", "hfh_slv_destroy", "hfh_slv_iterations", "hfh_slv_converged"):
+        getattr(L, f).argtypes = [vp]
+    L.hfh_slv_destroy.restype = None
+    L.hfh_slv_residual.argtypes = [vp]
+    L.hfh_slv_residual.restype = C.c_double
+    L.hfh_slv_set_tolerances.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.hfh_create.restype = vp
+    L.hfh_create.argtypes = [C.c_int]
+    L.hfh_destroy.argtypes = [vp]
+    L.hfh_destroy.restype = None
+    L.hfh_add_domain_amr.argtypes = [vp, cp]
+    L.hfh_add_bc_amr.argtypes = [vp, C.c_int, cp, C.c_int, ip]
+    L.hfh_setup.argtypes = [vp, C.POINTER(Params)]
+    L.hfh_bc_count.argtypes = [vp, C.c_int, C.c_int]
+    L.hfh_bc_centers.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.hfh_bc_set_values.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.hfh_field_size.argtypes = [vp, C.c_int, C.c_int]
+    for f in ("hfh_field_centers", "hfh_get_field", "hfh_set_field"):
+        getattr(L, f).argtypes = [vp, C.c_int, C.c_int, dp]
+    L.hfh_solver_step.argtypes = [vp]
+    L.hfh_solver_step_viscoelastic.argtypes = [vp]
+    L.hfh_set_time.argtypes = [vp, C.c_double]
+    L.hfh_get_time.argtypes = [vp]
+    L.hfh_get_time.restype = C.c_double
+    L.hfh_pinned_cell.argtypes = [vp]
+    L.hfh_last_solve.argtypes = [vp, ip, dp, ip]
+    L.hfh_stencil.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp, dp]
+    L.hfh_timing.argtypes = [vp, dp, lp]
+    L.hfh_reset_timing.argtypes = [vp]
+    L.hfh_stage_bytes.argtypes = [vp, C.c_int]
+    L.hfh_stage_bytes.restype = C.c_long
+    L.hfh_sync.argtypes = [vp]
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(L, rc, what):
+    if rc < 0:
+        raise HipError("%s failed (%d): %s" % (what, rc, L.hfh_last_error().decode()))
+    return rc
+
+
+def params_from(case_params, **over):
+    p = Params()
+    d = dict(case_params)
+    d.update(over)
+    for name, _ in Params._fields_:
+        if name in d:
+            setattr(p, name, d[name])
+    return p
+
+
+class Flow:
+    """Device-resident HiG-Flow solver built from a case dict
+    (``higflow_amd.cases``); mirrors higflow_solver_step(_viscoelastic)."""
+
+    def __init__(self, case, **param_over):
+        self.L = load()
+        self.dim = case["dim"]
+        self.case = case
+        self.h = self.L.hfh_create(self.dim)
+        if not self.h:
+            raise HipError("hfh_create: " + self.L.hfh_last_error().decode())
+        for amr in case["domains"]:
+            _check(self.L, self.L.hfh_add_domain_amr(self.h, amr.encode()), "hfh_add_domain_amr")
+        for bc in case["bcs"]:
+            ut = (C.c_int * self.dim)(*bc["utypes"])
+            _check(self.L, self.L.hfh_add_bc_amr(self.h, bc["id"], bc["amr"].encode(), bc["ptype"], ut),
+                   "hfh_add_bc_amr")
+        self.params = params_from(case["params"], **param_over)
+        _check(self.L, self.L.hfh_setup(self.h, C.byref(self.params)), "hfh_setup")
+        for i, bc in enumerate(case["bcs"]):
+            for kind in range(self.dim + 1):
+                xyz = self.bc_centers(kind, i)
+                vals = np.ascontiguousarray([bc["value"](kind, x) for x in xyz], dtype=np.float64)
+                _check(self.L, self.L.hfh_bc_set_values(self.h, kind, i, _dp(vals)), "hfh_bc_set_values")
+        self.ve = case["params"].get("flowtype", 0) == 3
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hfh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bc_centers(self, kind, i):
+        n = _check(self.L, self.L.hfh_bc_count(self.h, kind, i), "hfh_bc_count")
+        out = np.zeros((n, self.dim))
+        if n:
+            self.L.hfh_bc_centers(self.h, kind, i, _dp(out))
+        return out
+
+    def size(self, field, comp=0):
+        return _check(self.L, self.L.hfh_field_size(self.h, field, comp), "hfh_field_size")
+
+    def centers(self, field, comp=0):
+        out = np.zeros((self.size(field, comp), self.dim))
+        self.L.hfh_field_centers(self.h, field, comp, _dp(out))
+        return out
+
+    def get(self, field, comp=0):
+        out = np.zeros(self.size(field, comp))
+        _check(self.L, self.L.hfh_get_field(self.h, field, comp, _dp(out)), "hfh_get_field")
+        return out
+
+    def set(self, field, comp, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self.size(field, comp):
+            raise ValueError("field size mismatch")
+        _check(self.L, self.L.hfh_set_field(self.h, field, comp, _dp(v)), "hfh_set_field")
+
+    def step(self):
+        if self.ve:
+            return _check(self.L, self.L.hfh_solver_step_viscoelastic(self.h), "hfh_solver_step_viscoelastic")
+        return _check(self.L, self.L.hfh_solver_step(self.h), "hfh_solver_step")
+
+    def advance(self, n):
+        rc = 0
+        for _ in range(n):
+            rc = max(rc, self.step())
+        return rc
+
+    def sync(self):
+        _check(self.L, self.L.hfh_sync(self.h), "hfh_sync")
+
+    def time(self):
+        return self.L.hfh_get_time(self.h)
+
+    def pinned_cell(self):
+        return self.L.hfh_pinned_cell(self.h)
+
+    def last_solve(self):
+        it, outer = C.c_int(0), C.c_int(0)
+        r = C.c_double(0)
+        conv = self.L.hfh_last_solve(self.h, C.byref(it), C.byref(r), C.byref(outer))
+        return conv, it.value, r.value, outer.value
+
+    def stencil(self, domain, dim, center, x, alpha=1.0):
+        ids = (C.c_int * 4096)()
+        w = np.zeros(4096)
+        rhs = C.c_double(0)
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        n = _check(self.L, self.L.hfh_stencil(self.h, domain, dim, _dp(c), _dp(xx), alpha, 4096, ids,
+                                              _dp(w), C.byref(rhs)), "hfh_stencil")
+        return np.array(ids[:n], dtype=np.int64), w[:n].copy(), rhs.value
+
+    def timing(self):
+        ms = np.zeros(len(STAGES))
+        calls = (C.c_long * len(STAGES))()
+        self.L.hfh_timing(self.h, _dp(ms), calls)
+        return {s: (ms[i], calls[i]) for i, s in enumerate(STAGES)}
+
+    def reset_timing(self):
+        self.L.hfh_reset_timing(self.h)
+
+    def stage_bytes(self):
+        return {s: self.L.hfh_stage_bytes(self.h, i) for i, s in enumerate(STAGES)}
+
+
+class LinearSolver:
+    """The HIP solver-plugin engine: the _solver_vtable members of
+    higtree/src/solver.h:42-60 (slv_* semantics)."""
+
+    def __init__(self, first_gid, size):
+        self.L = load()
+        self.h = self.L.hfh_slv_create(first_gid, size)
+        if not self.h:
+            raise HipError("hfh_slv_create: " + self.L.hfh_last_error().decode())
+        self.n = size
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hfh_slv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_Ai(self, i, cols, vals):
+        cols = (C.c_int * len(cols))(*[int(c) for c in cols])
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        _check(self.L, self.L.hfh_slv_set_Ai(self.h, i, len(v), cols, _dp(v)), "hfh_slv_set_Ai")
+
+    def set_bi(self, i, v):
+        _check(self.L, self.L.hfh_slv_set_bi(self.h, i, v), "hfh_slv_set_bi")
+
+    def add_bi(self, i, v):
+        _check(self.L, self.L.hfh_slv_add_bi(self.h, i, v), "hfh_slv_add_bi")
+
+    def set_xi(self, i, v):
+        _check(self.L, self.L.hfh_slv_set_xi(self.h, i, v), "hfh_slv_set_xi")
+
+    def impose_value(self, line, v):
+        _check(self.L, self.L.hfh_slv_impose_value(self.h, line, v), "hfh_slv_impose_value")
+
+    def set_tolerances(self, rtol, atol, maxit):
+        _check(self.L, self.L.hfh_slv_set_tolerances(self.h, rtol, atol, maxit), "hfh_slv_set_tolerances")
+
+    def assemble(self):
+        for f in ("hfh_slv_assemble_rhs", "hfh_slv_assemble_output", "hfh_slv_assemble_matrix"):
+            _check(self.L, getattr(self.L, f)(self.h), f)
+
+    def solve(self):
+        _check(self.L, self.L.hfh_slv_solve(self.h), "hfh_slv_solve")
+
+    def get_x(self):
+        out = np.zeros(self.n)
+        _check(self.L, self.L.hfh_slv_get_x(self.h, _dp(out)), "hfh_slv_get_x")
+        return out
+
+    def stats(self):
+        return (self.L.hfh_slv_converged(self.h), self.L.hfh_slv_iterations(self.h),
+                self.L.hfh_slv_residual(self.h))

@@ -82,6 +82,11 @@ int hfo_wls_weights(int dim, int ord, int maxpts, int npts, const double *pts,
 int hfo_last_solve(hfo_solver *s, int *iters, double *resid);
 int hfo_pinned_cell(hfo_solver *s);
 
+/* test hooks: cyclic Jacobi (row-major DIM x DIM) and the convection-free
+ * constitutive right-hand side dK/dt of one cell */
+int hfo_jacobi(const double *A, double *lam, double *R);
+int hfo_ve_local_rhs(const hfo_params *par, const double *Du, const double *K, double *rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,8 @@ def lib(dim, variant=""):
     L.hfo_wls_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp]
     L.hfo_last_solve.argtypes = [vp, ip, dp]
     L.hfo_pinned_cell.argtypes = [vp]
+    L.hfo_jacobi.argtypes = [dp, dp, dp]
+    L.hfo_ve_local_rhs.argtypes = [C.POINTER(Params), dp, dp, dp]
     _LIBS[key] = L
     return L
 
@@ -188,3 +190,24 @@ def wls_weights(dim, ord_, maxpts, pts, x, libdim=2):
     if n < 0:
         raise ValueError("bad wls input")
     return w
+
+
+def jacobi(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    d = A.shape[0]
+    L = lib(d, "")
+    lam = np.zeros(d)
+    R = np.zeros((d, d))
+    L.hfo_jacobi(_dp(A), _dp(lam), _dp(R))
+    return lam, R
+
+
+def ve_local_rhs(params, Du, K):
+    Du = np.ascontiguousarray(Du, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    d = Du.shape[0]
+    L = lib(d, "")
+    out = np.zeros((d, d))
+    p = params_from(params)
+    L.hfo_ve_local_rhs(C.byref(p), _dp(Du), _dp(K), _dp(out))
+    return out

@@ -1544,3 +1544,72 @@ int hfo_last_solve(hfo_solver *s, int *iters, double *resid)
 }
 
 int hfo_pinned_cell(hfo_solver *s) { return s->desingpressure ? s->pin : -1; }
+
+/* Test hooks: the cyclic Jacobi eigen-solver and the local (convection-free)
+ * constitutive right-hand side of one cell, for closed-form checks. */
+int hfo_jacobi(const double *A, double *lam, double *R)
+{
+    real a[DIM][DIM], v[DIM][DIM], d[DIM];
+    for (int i = 0; i < DIM; i++)
+        for (int j = 0; j < DIM; j++) a[i][j] = A[i * DIM + j];
+    jacobi(a, d, v);
+    for (int i = 0; i < DIM; i++) {
+        lam[i] = d[i];
+        for (int j = 0; j < DIM; j++) R[i * DIM + j] = v[i][j];
+    }
+    return 0;
+}
+
+int hfo_ve_local_rhs(const hfo_params *par, const double *Du, const double *K, double *rhs_out)
+{
+    hfo_solver tmp;
+    memset(&tmp, 0, sizeof tmp);
+    tmp.par = *par;
+    hfo_solver *s = &tmp;
+    real D_[DIM][DIM], Kn[DIM][DIM], Kc[DIM][DIM];
+    for (int i = 0; i < DIM; i++)
+        for (int j = 0; j < DIM; j++) {
+            D_[i][j] = Du[i * DIM + j];
+            Kn[i][j] = K[i * DIM + j];
+            Kc[i][j] = Kn[i][j];
+        }
+    real R[DIM][DIM], Kl[DIM], lam[DIM];
+    jacobi(Kc, Kl, R);
+    for (int k = 0; k < DIM; k++) lam[k] = kernel_inverse(s, Kl[k]);
+    real M[DIM][DIM];
+    mat_rt_a_r(D_, R, M);
+    real Oa[DIM][DIM], Om[DIM][DIM];
+    for (int i = 0; i < DIM - 1; i++) {
+        for (int j = i + 1; j < DIM; j++) {
+            Oa[i][j] = (M[i][j] * lam[j] + M[j][i] * lam[i]) / (lam[j] - lam[i] + EPSMACH);
+            Oa[j][i] = -Oa[i][j];
+        }
+        Oa[i][i] = 0.0;
+    }
+    Oa[DIM - 1][DIM - 1] = 0.0;
+    mat_r_a_rt(Oa, R, Om);
+    real jl[DIM], Ba[DIM][DIM], BB[DIM][DIM], Ma[DIM][DIM], MM[DIM][DIM];
+    for (int k = 0; k < DIM; k++) jl[k] = kernel_jacobian(s, lam[k]);
+    for (int i = 0; i < DIM; i++) {
+        for (int j = 0; j < DIM; j++) {
+            Ba[i][j] = 0.0;
+            Ma[i][j] = 0.0;
+        }
+        Ba[i][i] = M[i][i] * lam[i] * jl[i];
+        real aux = 1.0 - lam[i];
+        Ma[i][i] = (s->par.ve_model == GIESEKUS) ? (aux - s->par.giesekus_alpha * aux * aux) * jl[i]
+                                                 : (1.0 - lam[i]) * jl[i];
+    }
+    mat_r_a_rt(Ba, R, BB);
+    mat_r_a_rt(Ma, R, MM);
+    for (int i = 0; i < DIM; i++)
+        for (int j = 0; j < DIM; j++) {
+            real ok = 0.0, ko = 0.0;
+            for (int k = 0; k < DIM; k++) {
+                ok += Om[i][k] * Kn[k][j];
+                ko += Kn[i][k] * Om[k][j];
+            }
+            rhs_out[i * DIM + j] = ok - ko + 2.0 * BB[i][j] + MM[i][j] / s->par.De;
+        }
+    return 0;
+}

@@ -1,0 +1,106 @@
+"""GPU parity of the HIP projection step against the CPU oracle on the same
+seeded inputs (FP64 form: 1e-10 relative), through the C ABI."""
+import numpy as np
+import pytest
+
+import hfo
+from higflow_amd import cases, native
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(rtol=1e-13, atol=1e-15, maxit=50000)
+
+
+def rel(a, b):
+    s = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / s
+
+
+def fields(sim, dim, ve=False):
+    out = {("u", d): sim.get(0, d) for d in range(dim)}
+    out["p"] = sim.get(1)
+    if ve:
+        for m in range(dim * dim):
+            out[("K", m)] = sim.get(6, m)
+            out[("S", m)] = sim.get(7, m)
+    return out
+
+
+def compare(case, steps, tol=1e-10, init=None):
+    o = hfo.Oracle(case)
+    h = native.Flow(case)
+    dim = case["dim"]
+    if init:
+        init(o)
+        init(h)
+    ve = case["params"]["flowtype"] == 3
+    worst = {}
+    for k in range(steps):
+        o.step()
+        h.step()
+    fo, fh = fields(o, dim, ve), fields(h, dim, ve)
+    for key in fo:
+        worst[key] = rel(fh[key], fo[key]) if np.abs(fo[key]).max() > 0 else np.abs(fh[key]).max()
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    assert not bad, bad
+    return worst
+
+
+def test_cavity_small_parity():
+    w = compare(cases.cavity(16, **TIGHT), 5)
+    print(w)
+
+
+def test_cavity_32_parity_20_steps():
+    compare(cases.cavity(32, **TIGHT), 20)
+
+
+@pytest.mark.slow
+def test_cavity_64_re100_100_steps():
+    # lid-driven cavity 64x64, Re=100, dt=1e-3, 100 explicit Euler steps
+    compare(cases.cavity(64, Re=100.0, dt=1e-3, **TIGHT), 100)
+
+
+def test_cavity_first_order_and_central():
+    compare(cases.cavity(16, convecdiscrtype=cases.FIRST_ORDER, **TIGHT), 4)
+    compare(cases.cavity(16, convecdiscrtype=cases.CENTRAL, **TIGHT), 4)
+
+
+def test_channel_from_reference_fixture():
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "newt_poiseuille.npz"))
+
+    def init(sim):
+        for name, f, c in (("u0", 0, 0), ("u1", 0, 1), ("p", 1, 0)):
+            sim.set(f, c, fx[name][:, 2])
+    compare(cases.newt_channel(**TIGHT), 2, init=init)
+
+
+def test_cavity_3d_parity():
+    compare(cases.cavity(8, dim=3, Re=100.0, **TIGHT), 3)
+
+
+def test_nonincremental_projection():
+    compare(cases.cavity(16, projtype=cases.NON_INCREMENTAL, **TIGHT), 3)
+
+
+@pytest.mark.parametrize("model,convec", [(cases.OLDROYD_B, cases.CELL_CUBISTA),
+                                          (cases.GIESEKUS, cases.CELL_CUBISTA),
+                                          (cases.OLDROYD_B, cases.CELL_CENTRAL)])
+def test_viscoelastic_cavity_parity(model, convec):
+    case = cases.ve_cavity(16, Re=1.0, Wi=1.0, beta=0.5, dt=2e-4, ve_model=model, ve_convec=convec, **TIGHT)
+    compare(case, 5)
+
+
+def test_mixed_precision_error_bound():
+    case = cases.cavity(64, Re=100.0, **TIGHT)
+    h64 = native.Flow(case)
+    hmx = native.Flow(case, precision=1, inner_rtol=1e-5)
+    for _ in range(20):
+        h64.step()
+        hmx.step()
+    for d in range(2):
+        a, b = hmx.get(0, d), h64.get(0, d)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+    a, b = hmx.get(1), h64.get(1)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6

@@ -1,0 +1,66 @@
+"""Step-level pins of the oracle: the reference-held channel fixture is a
+steady state of the restated discretisation (revision without the
+outside-Dirichlet ghost, as in the fixture), zero fields are a fixed point,
+and the projection leaves a divergence-free velocity."""
+import os
+
+import numpy as np
+
+import hfo
+from higflow_amd import cases
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "newt_poiseuille.npz")
+
+
+def test_fixture_is_a_steady_state_of_the_fixture_revision():
+    fx = np.load(GOLD)
+    o = hfo.Oracle(cases.newt_channel(rtol=1e-12, atol=1e-14), variant="_nog")
+    for name, field, comp in (("u0", 0, 0), ("u1", 0, 1), ("p", 1, 0)):
+        o.set(field, comp, fx[name][:, 2])
+    u0 = o.get(0, 0)
+    assert o.step() == 0
+    du = np.abs(o.get(0, 0) - u0).max()
+    assert du < 2e-7, du
+    assert np.abs(o.get(2)).max() < 1e-4
+
+
+def test_current_revision_differs_only_near_walls():
+    # the outside-Dirichlet ghost (domain.c:877-1029) is linear: the exact
+    # parabola is no longer steady in the first facet row
+    fx = np.load(GOLD)
+    o = hfo.Oracle(cases.newt_channel(rtol=1e-12, atol=1e-14))
+    for name, field, comp in (("u0", 0, 0), ("u1", 0, 1), ("p", 1, 0)):
+        o.set(field, comp, fx[name][:, 2])
+    us0 = o.get(0, 0)
+    o.step()
+    ustar = o.get(3, 0)
+    y = o.centers(0, 0)[:, 1]
+    inner = np.abs(y) < 0.9
+    assert np.abs(ustar - us0)[inner].max() < 1e-6
+    # first row: residual dt*(3 - 2.25) of the linear reflection
+    first = np.abs(np.abs(y) - 0.975) < 1e-9
+    assert abs(np.abs(ustar - us0)[first].max() - 1e-3 * 0.75) < 2e-5
+
+
+def test_zero_lid_is_a_fixed_point():
+    o = hfo.Oracle(cases.cavity(16, lid=0.0))
+    for _ in range(3):
+        o.step()
+    assert np.abs(o.get(0, 0)).max() == 0.0
+    assert np.abs(o.get(1)).max() == 0.0
+
+
+def test_projection_is_divergence_free():
+    n = 32
+    o = hfo.Oracle(cases.cavity(n, rtol=1e-13, atol=1e-15))
+    for _ in range(3):
+        o.step()
+    h = 1.0 / n
+    u = o.get(0, 0).reshape(n - 1, n)
+    v = o.get(0, 1).reshape(n, n - 1)
+    U = np.zeros((n + 1, n))
+    U[1:n] = u
+    V = np.zeros((n, n + 1))
+    V[:, 1:n] = v
+    div = (U[1:] - U[:-1]) / h + (V[:, 1:] - V[:, :-1]) / h
+    assert np.abs(div).max() < 1e-8
