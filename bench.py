@@ -1,0 +1,178 @@
+#!/usr/bin/env python
+"""Benchmark of the HiG-Flow projection time step on one MI355X.
+
+Metric: cell-updates/s per full time step, mixed precision
+versus the plain FP64 GPU step, plus the Poisson solve time per step.
+``value`` is the mixed-precision step's cell-updates/s (cells x steps / wall
+seconds of the timed steps, fields resident in HBM); the FP64 step, their
+ratio, the Poisson milliseconds and the mixed-vs-FP64 errors are reported
+beside it.  ``roofline`` is measured live for the dominant kernel with
+hipEvents on the library stream; ``cpu_baseline`` times the CPU oracle (a
+restatement of the reference algorithm) on the host on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+
+# workloads built today (see DESIGN.md for the configuration map)
+WORKLOADS = {
+    # 2D Newtonian lid-driven cavity, uniform 64x64, Re=100, dt=1e-3,
+    # explicit Euler, incremental projection, CUBISTA, Poisson 1e-10
+    "cavity2d_64_re100": dict(kind="cavity", n=64, Re=100.0, dt=1e-3),
+    # the same cavity on a 512x512 mesh (dt limited by the explicit viscous bound)
+    "cavity2d_256_re100": dict(kind="cavity", n=256, Re=100.0, dt=1e-4),
+    # 2D Oldroyd-B cavity (log-conformation), uniform 128x128
+    "ve_cavity2d_128": dict(kind="ve_cavity", n=128, Re=1.0, Wi=1.0, beta=0.5, dt=1e-5),
+}
+DEFAULT = "cavity2d_64_re100"
+
+
+def make_case(spec, **tol):
+    from higflow_amd import cases
+    if spec["kind"] == "cavity":
+        return cases.cavity(spec["n"], Re=spec["Re"], dt=spec["dt"], **tol)
+    return cases.ve_cavity(spec["n"], Re=spec["Re"], Wi=spec["Wi"], beta=spec["beta"], dt=spec["dt"], **tol)
+
+
+def run_gpu(case, steps, warmup, **over):
+    from higflow_amd import native
+    sim = native.Flow(case, **over)
+    for _ in range(warmup):
+        sim.step()
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    sim.sync()
+    wall = time.perf_counter() - t0
+    return sim, wall
+
+
+def stage_pass(sim, steps):
+    sim.reset_timing()
+    sim.set_timing(True)
+    for _ in range(steps):
+        sim.step()
+    sim.sync()
+    sim.set_timing(False)
+    return {k: v[0] / max(v[1], 1) for k, v in sim.timing().items()}
+
+
+def probe(sim, which, reps):
+    return sim.probe(which, reps)
+
+
+def cpu_baseline(case, budget_s=15.0, max_steps=400):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import hfo
+    o = hfo.Oracle(case)
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps and time.perf_counter() - t0 < budget_s:
+        o.step()
+        n += 1
+    wall = time.perf_counter() - t0
+    return case["ncells"] * n / wall, n, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the projection step on one GPU; multi-GPU runs are not part of it "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    spec = WORKLOADS[args.workload]
+    ref_tol = dict(rtol=1e-10, atol=1e-10, maxit=50000)
+    case = make_case(spec, **ref_tol)
+    ncells = case["ncells"]
+
+    sim64, wall64 = run_gpu(case, args.steps, args.warmup, precision=0)
+    simmx, wallmx = run_gpu(case, args.steps, args.warmup, precision=1, inner_rtol=1e-5)
+    cups64 = ncells * args.steps / wall64
+    cupsmx = ncells * args.steps / wallmx
+    errs = {}
+    for name, f, c in [("u%d" % d, 0, d) for d in range(case["dim"])] + [("p", 1, 0)]:
+        a, b = simmx.get(f, c), sim64.get(f, c)
+        nb = np.linalg.norm(b)
+        errs[name] = dict(l2=float(np.linalg.norm(a - b) / nb) if nb > 0 else 0.0,
+                          linf=float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
+    k = max(5, min(20, args.steps // 5))
+    st64 = stage_pass(sim64, k)
+    stmx = stage_pass(simmx, k)
+    conv, iters, resid, _ = sim64.last_solve()
+
+    # dominant kernel: the pressure SpMV (one launch per Krylov iteration)
+    pr = probe(sim64, 0, 400)
+    roof = None
+    if pr:
+        ms, by = pr
+        ach = by / (ms * 1e-3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+        if os.path.exists(tf):
+            traffic = json.load(open(tf)).get("spmv_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": traffic,
+                "kernel": "k_spmv_dot<double>", "bytes_per_launch": by, "avg_us": round(ms * 1e3, 3)}
+    kernels = {}
+    for which, name in ((1, "k_momentum"), (2, "k_interp(momentum rows)"), (3, "k_interp(divergence rows)"),
+                        (4, "k_interp(projection rows)")):
+        p = probe(sim64, which, 200)
+        if p:
+            kernels[name] = {"avg_us": round(p[0] * 1e3, 3), "bytes": p[1],
+                             "GBps": round(p[1] / (p[0] * 1e-3) / 1e9, 2)}
+    cpu = None
+    if not args.no_cpu:
+        v, n, wall = cpu_baseline(case)
+        cpu = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": "%d steps of %s on the CPU oracle (BiCGStab+ILU(0), rtol=atol=1e-10), %.1f s"
+                         % (n, args.workload, wall)}
+    line = {
+        "metric": "cell-updates/s per full time step (mixed vs FP64 GPU) + Poisson solve ms",
+        "value": round(cupsmx, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wallmx / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64-refinement/f32-inner",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "cells": ncells, "dim": case["dim"],
+                   "dt": case["params"]["dt"], "Re": case["params"]["Re"]},
+        "fp64_value": round(cups64, 1),
+        "fp64_ms_per_step": round(wall64 / args.steps * 1e3, 4),
+        "mixed_over_fp64": round(cupsmx / cups64, 3),
+        "poisson_ms_fp64": round(st64["solve"], 4),
+        "poisson_ms_mixed": round(stmx["solve"], 4),
+        "poisson_iters_fp64": iters,
+        "err_mixed_vs_fp64": errs,
+        "stage_ms_fp64": {k2: round(v2, 4) for k2, v2 in st64.items()},
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

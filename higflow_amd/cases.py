@@ -132,3 +132,76 @@ def newt_channel(nx=160, ny=40, **kw):
     params = default_params(Re=1.0, dt=1e-3, **kw)
     return dict(dim=2, domains=[dom], bcs=bcs, params=params, name="newt_channel",
                 ncells=nx * ny)
+
+
+def _contraction_amr(lo, hi, h0, levels, boxes):
+    """.amr text of one tree: base spacing h0 on [lo, hi], then `levels`
+    factor-2 refinement levels over the cells inside any of `boxes`
+    ((xlo, xhi, ylo, yhi) in absolute coordinates).  Patches follow the
+    reference's convention: level delta, 1-based first cell, cell counts."""
+    nx = int(round((hi[0] - lo[0]) / h0))
+    ny = int(round((hi[1] - lo[1]) / h0))
+    lines = ["%s %s %s %s" % (_fmt(lo[0]), _fmt(hi[0]), _fmt(lo[1]), _fmt(hi[1])), str(1 + levels),
+             "%s %s 1" % (_fmt(h0), _fmt(h0)), "1 1 %d %d" % (nx, ny)]
+    for lv in range(1, levels + 1):
+        h = h0 / (2 ** lv)
+        patches = []
+        for (bx0, bx1, by0, by1) in boxes:
+            x0, x1 = max(bx0, lo[0]), min(bx1, hi[0])
+            y0, y1 = max(by0, lo[1]), min(by1, hi[1])
+            if x1 <= x0 or y1 <= y0:
+                continue
+            i0 = int(round((x0 - lo[0]) / h))
+            j0 = int(round((y0 - lo[1]) / h))
+            ni = int(round((x1 - x0) / h))
+            nj = int(round((y1 - y0) / h))
+            patches.append("%d %d %d %d" % (i0 + 1, j0 + 1, ni, nj))
+        lines.append("%s %s %d" % (_fmt(h), _fmt(h), len(patches)))
+        lines.extend(patches)
+    return "\n".join(lines) + "\n"
+
+
+def contraction(levels=3, h0=0.1, length=20.0, H=4.0, h=1.0, radius=1.0, Re=0.01, Wi=1.0,
+                beta=1.0 / 9.0, dt=1e-4, viscoelastic=True, **kw):
+    """Planar 4:1 contraction: upstream channel [-L/2, 0] x [-H, H], downstream
+    [0, L/2] x [-h, h] as two HiG-Tree trees (the layout of
+    higflow/example2d_Newt_contraction), base spacing h0 and `levels` levels
+    of refinement over the boxes within `radius` of the re-entrant corners
+    (0, +-h).  Parabolic inflow with mean 1, no-slip walls, outflow with
+    velocity Neumann / pressure Dirichlet.  Oldroyd-B, log-conformation."""
+    Lh = length / 2.0
+    boxes = [(-radius, radius, h - radius, h + radius), (-radius, radius, -h - radius, -h + radius)]
+    up = _contraction_amr((-Lh, -H), (0.0, H), h0, levels, boxes)
+    dn = _contraction_amr((0.0, -h), (Lh, h), h0, levels, boxes)
+    nyu, nyd, nx = int(round(2 * H / h0)), int(round(2 * h / h0)), int(round(Lh / h0))
+    nw = int(round((H - h) / h0))
+
+    def wall(amr):
+        return dict(amr=amr, ptype=NEUMANN, utypes=[DIRICHLET, DIRICHLET], value=lambda kind, x: 0.0)
+
+    def inflow(kind, x):
+        return 1.5 * (1.0 - (x[1] / H) ** 2) if kind == 1 else 0.0
+
+    f = _fmt
+    bcs = [
+        dict(amr="%s %s %s %s\n1\n0 %s 1\n1 1 1 %d\n" % (f(-Lh), f(-Lh), f(-H), f(H), f(h0), nyu),
+             ptype=NEUMANN, utypes=[DIRICHLET, DIRICHLET], value=inflow),
+        wall("%s 0.0 %s %s\n1\n%s 0 1\n1 1 %d 1\n" % (f(-Lh), f(H), f(H), f(h0), nx)),
+        wall("0.0 0.0 %s %s\n1\n0 %s 1\n1 1 1 %d\n" % (f(h), f(H), f(h0), nw)),
+        wall("0.0 %s %s %s\n1\n%s 0 1\n1 1 %d 1\n" % (f(Lh), f(h), f(h), f(h0), nx)),
+        dict(amr="%s %s %s %s\n1\n0 %s 1\n1 1 1 %d\n" % (f(Lh), f(Lh), f(-h), f(h), f(h0), nyd),
+             ptype=DIRICHLET, utypes=[NEUMANN, NEUMANN], value=lambda kind, x: 0.0),
+        wall("0.0 %s %s %s\n1\n%s 0 1\n1 1 %d 1\n" % (f(Lh), f(-h), f(-h), f(h0), nx)),
+        wall("0.0 0.0 %s %s\n1\n0 %s 1\n1 1 1 %d\n" % (f(-H), f(-h), f(h0), nw)),
+        wall("%s 0.0 %s %s\n1\n%s 0 1\n1 1 %d 1\n" % (f(-Lh), f(-H), f(-H), f(h0), nx)),
+    ]
+    for i, b in enumerate(bcs):
+        b["id"] = i
+    kw.setdefault("tempdiscrtype", SEMI_IMPLICIT_EULER)
+    if viscoelastic:
+        kw.setdefault("flowtype", VISCOELASTIC)
+        kw.setdefault("ve_model", OLDROYD_B)
+        kw.setdefault("kernel_transform", 1)
+    params = default_params(Re=Re, dt=dt, De=Wi, beta=beta, **kw)
+    return dict(dim=2, domains=[up, dn], bcs=bcs, params=params,
+                name="contraction_%dlev_h%s" % (levels, f(h0)), ncells=None)

@@ -396,6 +396,19 @@ int hfh_reset_timing(hfh_solver *s) {
     s->f->reset_timing();
     return 0;
 }
+int hfh_set_timing(hfh_solver *s, int on) {
+    HFH_SOLVER(s);
+    s->f->set_timing(on != 0);
+    return 0;
+}
+int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes) {
+    HFH_SOLVER(s);
+    return guard([&] {
+        int r = s->f->probe(which, reps, avg_ms, bytes);
+        if (r < 0) g_err = "hfh_probe: unknown kernel id for this case, or no device";
+        return r;
+    });
+}
 long hfh_stage_bytes(hfh_solver *s, int stage) { return (s && s->f) ? s->f->bytes(stage) : -1; }
 int hfh_sync(hfh_solver *s) {
     HFH_SOLVER(s);

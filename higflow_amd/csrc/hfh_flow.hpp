@@ -64,6 +64,8 @@ class FlowBase {
     virtual void timing(double *ms_per_stage, long *calls) = 0;
     virtual void reset_timing() = 0;
     virtual long bytes(int stage) const = 0;
+    virtual void set_timing(bool on) = 0;
+    virtual int probe(int which, int reps, double *avg_ms, long *bytes) = 0;
     virtual void sync() = 0;
 };
 
@@ -145,6 +147,8 @@ class Flow : public FlowBase {
         }
     }
     long bytes(int stage) const override { return stage >= 0 && stage < ST_NSTAGE ? bytes_[stage] : 0; }
+    void set_timing(bool on) override { timing_on_ = on; }
+    int probe(int which, int reps, double *avg_ms, long *bytes) override;
     void sync() override { HFH_CHECK(hipStreamSynchronize(st_)); }
 
   private:
@@ -174,6 +178,12 @@ class Flow : public FlowBase {
     bool symmetric_ = true;
     std::vector<int> Arp_, Aci_;
     std::vector<double> Av_;
+    // semi-implicit Euler: Helmholtz rows per velocity component
+    std::vector<int> Hrp_[D], Hci_[D];
+    std::vector<double> Hv_[D];
+    std::vector<double> Hrhs0_[D];
+    std::vector<std::vector<std::pair<int, double>>> Hbc_[D];
+    bool Hsym_[D] = {};
 
     // ---------------------------------------------------------- device
     hipStream_t st_ = nullptr;
@@ -188,8 +198,11 @@ class Flow : public FlowBase {
     DevBuf<unsigned> dMomFlags_[D], dVeFlags_;
     DevBuf<double> dMomDelta_[D], dVeDelta_;
     DeviceSystem psys_;
-    SolveStats stats_;
+    DeviceSystem hsys_[D];
+    DevBuf<double> hb_[D], hcst_[D];
+    SolveStats stats_, hstats_[D];
     hipEvent_t ev0_[ST_NSTAGE] = {}, ev1_[ST_NSTAGE] = {};
+    bool timing_on_ = false;
     double ms_[ST_NSTAGE] = {0};
     long calls_[ST_NSTAGE] = {0};
     long bytes_[ST_NSTAGE] = {0};
@@ -241,6 +254,7 @@ class Flow : public FlowBase {
     void refold_constants();
     void run_group(DGroup &dg, double alpha, double *dst, int stage);
     void tbegin(int s) {
+        if (!timing_on_) return;
         if (!ev0_[s]) {
             HFH_CHECK(hipEventCreate(&ev0_[s]));
             HFH_CHECK(hipEventCreate(&ev1_[s]));
@@ -248,6 +262,7 @@ class Flow : public FlowBase {
         HFH_CHECK(hipEventRecord(ev0_[s], st_));
     }
     void tend(int s) {
+        if (!timing_on_) return;
         HFH_CHECK(hipEventRecord(ev1_[s], st_));
         HFH_CHECK(hipEventSynchronize(ev1_[s]));
         float m = 0;
@@ -472,6 +487,7 @@ void Flow<D>::record_momentum(bool ve) {
             }
             flags |= (unsigned)ctype & 3u;
             momFlags_[dim][f] = flags;
+            bool semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
             if (ve) {  // hig-flow-discret.c:541-569
                 int tk = 2 + 6 * D + (D - 1);
                 for (int d2 = 0; d2 < D; d2++) {
@@ -508,6 +524,38 @@ void Flow<D>::record_momentum(bool ve) {
                     }
                     S(tk + d2) = -(1 + combo_row(gMom_, cs, -1));
                 }
+            }
+            if (semi) {  // hig-flow-step.c:826-853: (I - dt/Re L) row at this facet
+                Stencil H;
+                double alpha = 0.0;
+                for (int d2 = 0; d2 < D; d2++) {
+                    double w = -par_.dt / (par_.Re * fd[d2] * fd[d2]);
+                    alpha -= 2.0 * w;
+                    Vec<D> p = fc;
+                    p[d2] = fc[d2] + fd[d2];
+                    fu.get_stencil(fc, p, w, H);
+                    p[d2] = fc[d2] - fd[d2];
+                    fu.get_stencil(fc, p, w, H);
+                }
+                alpha = 1.0 + alpha;
+                fu.get_stencil(fc, fc, alpha, H);
+                if (f == 0) {
+                    Hrp_[dim].assign(1, 0);
+                    Hci_[dim].clear();
+                    Hv_[dim].clear();
+                    Hrhs0_[dim].clear();
+                    Hbc_[dim].clear();
+                }
+                std::vector<std::pair<int, double>> row;
+                for (size_t k = 0; k < H.ids.size(); k++) row.emplace_back(H.ids[k], H.w[k]);
+                std::sort(row.begin(), row.end());
+                for (auto &e : row) {
+                    Hci_[dim].push_back(e.first);
+                    Hv_[dim].push_back(e.second);
+                }
+                Hrp_[dim].push_back((int)Hci_[dim].size());
+                Hrhs0_[dim].push_back(H.rhs0);
+                Hbc_[dim].push_back(H.bc);
             }
         }
     }

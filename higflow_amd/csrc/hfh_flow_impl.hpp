@@ -130,6 +130,23 @@ void Flow<D>::finalize_layout() {
     bytes_[ST_PROJECT] = gbytes(gGrad_) + (long)gGrad_.nrows() * 12 + (long)Nc_ * 24;
     bytes_[ST_SOLVE] = (long)Aci_.size() * 12 + (long)Nc_ * 8 * 9;  // one CG iteration
     bytes_[ST_VE] = (long)ne * (D * D * 8 * 2 + NSYM * 8 * 2 + 6 * D * (4 + 8) + 4 + D * 8);
+    if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
+        for (int d = 0; d < D; d++) {
+            std::vector<std::map<int, double>> m(Nf_[d]);
+            for (int r = 0; r < Nf_[d]; r++)
+                for (int k = Hrp_[d][r]; k < Hrp_[d][r + 1]; k++) m[r][Hci_[d][k]] = Hv_[d][k];
+            Hsym_[d] = true;
+            for (int r = 0; r < Nf_[d] && Hsym_[d]; r++)
+                for (auto &e : m[r]) {
+                    auto it = m[e.first].find(r);
+                    double o = it == m[e.first].end() ? 0.0 : it->second;
+                    if (std::fabs(o - e.second) > 1e-12 * std::max(1.0, std::fabs(e.second))) {
+                        Hsym_[d] = false;
+                        break;
+                    }
+                }
+        }
+    }
     if (host_only_) return;
     // device upload
     std::vector<double> hx(nX_, 0.0);
@@ -166,6 +183,13 @@ void Flow<D>::finalize_layout() {
         dGrad_.y.upload(y.data(), y.size(), st_);
     }
     psys_.set_matrix(Arp_, Aci_, Av_, st_);
+    if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
+        for (int d = 0; d < D; d++) {
+            hsys_[d].set_matrix(Hrp_[d], Hci_[d], Hv_[d], st_);
+            hb_[d].alloc(Nf_[d]);
+            hcst_[d].alloc(Nf_[d]);
+        }
+    }
     refold_constants();
     HFH_CHECK(hipStreamSynchronize(st_));
 }
@@ -236,6 +260,17 @@ void Flow<D>::refold_constants() {
         std::vector<double> c = p.g->constants(bcval_);
         p.d->cst.upload(c.data(), c.size(), st_);
     }
+    if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
+        for (int d = 0; d < D; d++) {
+            std::vector<double> c(Nf_[d]);
+            for (int f = 0; f < Nf_[d]; f++) {
+                double r = Hrhs0_[d][f];
+                for (auto &b : Hbc_[d][f]) r += b.second * bcval_[b.first];
+                c[f] = r;
+            }
+            if (Nf_[d] > 0) hcst_[d].upload(c.data(), c.size(), st_);
+        }
+    }
     HFH_CHECK(hipStreamSynchronize(st_));
     bc_dirty_ = false;
 }
@@ -254,7 +289,8 @@ void Flow<D>::run_group(DGroup &dg, double alpha, double *dst, int stage) {
 template <int D>
 int Flow<D>::step(bool ve) {
     if (!ready_ || host_only_) return -1;
-    if (par_.tempdiscrtype != EXPLICIT_EULER) return -3;
+    if (par_.tempdiscrtype != EXPLICIT_EULER && par_.tempdiscrtype != SEMI_IMPLICIT_EULER) return -3;
+    const bool semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
     if (ve != (par_.flowtype == VISCOELASTIC)) return -4;
     if (bc_dirty_) refold_constants();
     double *X = X_.p;
@@ -300,9 +336,24 @@ int Flow<D>::step(bool ve) {
         mp.second = par_.secondconvecdiscrtype;
         mp.ve = ve;
         mp.dim = d;
+        mp.semi = semi;
+        mp.bout = semi ? hb_[d].p : nullptr;
+        mp.bcst = semi ? hcst_[d].p : nullptr;
         if (mp.nf > 0)
             k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[d].p, dMomFlags_[d].p, dMomDelta_[d].p,
                                                              X);
+    }
+    if (semi) {  // one Helmholtz solve per component, warm-started from the last u*
+        for (int d = 0; d < D; d++) {
+            if (Nf_[d] == 0) continue;
+            double *x = X + oUS_[d];
+            if (par_.precision == 1)
+                hstats_[d] = hsys_[d].mixed(hb_[d].p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
+            else if (Hsym_[d])
+                hstats_[d] = hsys_[d].pcg(hb_[d].p, x, par_.rtol, par_.atol, par_.maxit, st_);
+            else
+                hstats_[d] = hsys_[d].bicgstab(hb_[d].p, x, par_.rtol, par_.atol, par_.maxit, st_);
+        }
     }
     tend(ST_MOMENTUM);
     tbegin(ST_DIV);
@@ -324,6 +375,100 @@ int Flow<D>::step(bool ve) {
     tend(ST_PROJECT);
     par_.t += par_.dt;
     return stats_.converged ? 0 : 1;
+}
+
+// Launches one kernel of the step `reps` times back to back on the library
+// stream between two hipEvents (inputs as left by the last step):
+// Every probed kernel recomputes its outputs from inputs it does not write,
+// so repeated launches leave the state unchanged.
+//   0 pressure SpMV + partial dot (one CG iteration's operator pass)
+//   1 fused momentum kernel, first velocity component
+//   2 momentum interpolation rows
+//   3 cell divergence rows
+//   4 projection rows
+//   5 fused constitutive kernel (viscoelastic cases)
+template <int D>
+int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
+    if (!ready_ || host_only_ || reps <= 0) return -1;
+    hipEvent_t a, b;
+    HFH_CHECK(hipEventCreate(&a));
+    HFH_CHECK(hipEventCreate(&b));
+    double *X = X_.p;
+    long by = 0;
+    int n = Nc_;
+    auto gb = [&](const Group &g) { return (long)g.col.size() * 12 + (long)g.nrows() * (8 + 8 + 4 + 4) * g.ncomp; };
+    MomParams mp;
+    mp.nf = Nf_[0];
+    mp.ns = momNs_;
+    mp.u_off = oU_[0];
+    mp.us_off = oUS_[0];
+    mp.dt = par_.dt;
+    mp.Re = par_.Re;
+    mp.incremental = par_.projtype == INCREMENTAL;
+    mp.second = par_.secondconvecdiscrtype;
+    mp.ve = par_.flowtype == VISCOELASTIC;
+    mp.dim = 0;
+    mp.semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
+    mp.bout = mp.semi ? hb_[0].p : nullptr;
+    mp.bcst = mp.semi ? hcst_[0].p : nullptr;
+    HFH_CHECK(hipEventRecord(a, st_));
+    for (int r = 0; r < reps; r++) {
+        switch (which) {
+        case 0:
+            k_spmv_dot<double><<<psys_.nb, BLOCK, 0, st_>>>(n, psys_.rp.p, psys_.ci.p, psys_.v.p, psys_.p.p,
+                                                            psys_.q.p, psys_.part1.p);
+            by = (long)psys_.nnz * 12 + (long)(n + 1) * 4 + (long)n * 16 + (long)psys_.nb * 8;
+            break;
+        case 1:
+            k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[0].p, dMomFlags_[0].p,
+                                                             dMomDelta_[0].p, X);
+            by = (long)mp.nf * ((long)momNs_ * (4 + 8) + 4 + D * 8 + 8 + 8);
+            break;
+        case 2:
+            run_group(dMom_, 1.0, X, ST_INTERP);
+            by = gb(gMom_);
+            break;
+        case 3:
+            run_group(dDiv_, 1.0, bvec_.p, ST_DIV);
+            by = gb(gDiv_);
+            break;
+        case 4:
+            // recomputes u = ustar - dt*grad(dp) from unchanged inputs: idempotent
+            run_group(dGrad_, -par_.dt, X, ST_PROJECT);
+            by = gb(gGrad_) + (long)gGrad_.nrows() * 12;
+            break;
+        default: {
+            if (par_.flowtype != VISCOELASTIC) return -1;
+            VeParams vp;
+            vp.ne = sdED_->size();
+            vp.nrowK = gK_.nrows();
+            vp.oDv = oDv_;
+            vp.oK = oK_;
+            vp.oKN = oKN_;
+            vp.oS = oS_;
+            vp.dt = par_.dt;
+            vp.Re = par_.Re;
+            vp.De = par_.De;
+            vp.beta = par_.beta;
+            vp.tol = par_.kernel_tol;
+            vp.galpha = par_.giesekus_alpha;
+            vp.model = par_.ve_model;
+            vp.convec = par_.ve_convec;
+            vp.transform = par_.kernel_transform;
+            k_constitutive<D><<<nblocks(vp.ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
+            by = bytes_[ST_VE];
+        }
+        }
+    }
+    HFH_CHECK(hipEventRecord(b, st_));
+    HFH_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HFH_CHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *avg_ms = ms / reps;
+    *bytes = by;
+    return 0;
 }
 
 // field ids as the oracle: 0 u, 1 p, 2 dp, 3 ustar, 4 F, 5 FU, 6 kernel

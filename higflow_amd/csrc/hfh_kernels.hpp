@@ -43,7 +43,9 @@ struct MomParams {
     int u_off, us_off;
     double dt, Re;
     int incremental, second, ve, dim;
-    double delta[3];  // unused when per-facet deltas are given
+    int semi;            // semi-implicit Euler: write the Helmholtz right-hand side
+    double *bout;        // semi: b[f] = explicit part + boundary constant
+    const double *bcst;  // semi: boundary constant of each Helmholtz row
 };
 
 // Per-facet slot layout (produced by the recorder):
@@ -181,6 +183,20 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
     diff /= P.Re;
     double rhs = 0.0;
     double pt = P.incremental ? dpdx : 0.0;
+    if (P.semi) {  // hig-flow-step.c:815-824, hig-flow-step-viscoelastic.c:1137-1148
+        rhs += F;
+        rhs -= pt;
+        if (P.ve) {
+            int tk = 2 + 6 * D + (D - 1);
+            for (int d2 = 0; d2 < D; d2++) tens += X[slots[(size_t)(tk + d2) * nf + f]];
+            rhs += tens;
+        }
+        rhs -= conv;
+        rhs *= P.dt;
+        rhs += uf;
+        P.bout[f] = rhs + P.bcst[f];
+        return;
+    }
     if (!P.ve) {
         rhs -= pt;
         rhs -= conv;

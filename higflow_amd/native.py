@@ -50,6 +50,7 @@ def load():
     vp, dp, ip, cp, lp = (C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_char_p,
                           C.POINTER(C.c_long))
     L.hfh_last_error.restype = C.c_char_p
+    L.hfh_last_error.argtypes = []
     L.hfh_slv_create.restype = vp
     L.hfh_slv_create.argtypes = [C.c_int, C.c_int]
     L.hfh_slv_set_maxnonzeros.argtypes = [vp, C.c_int]
@@ -94,6 +95,11 @@ def load():
     L.hfh_stage_bytes.argtypes = [vp, C.c_int]
     L.hfh_stage_bytes.restype = C.c_long
     L.hfh_sync.argtypes = [vp]
+    L.hfh_set_timing.argtypes = [vp, C.c_int]
+    L.hfh_probe.argtypes = [vp, C.c_int, C.c_int, dp, lp]
+    L.hfh_version.argtypes = []
+    L.hfh_device_count.argtypes = []
+    L.hfh_create.argtypes = [C.c_int]
     _LIB = L
     return L
 
@@ -225,6 +231,18 @@ class Flow:
 
     def reset_timing(self):
         self.L.hfh_reset_timing(self.h)
+
+    def set_timing(self, on):
+        _check(self.L, self.L.hfh_set_timing(self.h, 1 if on else 0), "hfh_set_timing")
+
+    def probe(self, which, reps):
+        """(avg ms per launch, algorithmic bytes per launch) of one step kernel."""
+        ms = C.c_double(0)
+        by = C.c_long(0)
+        rc = self.L.hfh_probe(self.h, which, reps, C.byref(ms), C.byref(by))
+        if rc < 0:
+            return None
+        return ms.value, by.value
 
     def stage_bytes(self):
         return {s: self.L.hfh_stage_bytes(self.h, i) for i, s in enumerate(STAGES)}

@@ -148,6 +148,14 @@ int hfh_stencil(hfh_solver *s, int domain, int dim, const double *center, const 
 int hfh_timing(hfh_solver *s, double *ms, long *calls);
 int hfh_reset_timing(hfh_solver *s);
 long hfh_stage_bytes(hfh_solver *s, int stage);
+/* per-stage event timing inside hfh_solver_step* (off by default: it adds a
+ * host synchronisation per stage) */
+int hfh_set_timing(hfh_solver *s, int on);
+/* launches one step kernel `reps` times between hipEvents on the library
+ * stream: 0 pressure SpMV (+dot), 1 momentum, 2 momentum interpolation rows,
+ * 3 divergence rows, 4 projection rows, 5 constitutive.  Average ms per
+ * launch and the algorithmic bytes of one launch. */
+int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
 int hfh_sync(hfh_solver *s);
 
 #ifdef __cplusplus

@@ -63,6 +63,7 @@ struct hfo_solver {
     hstencil stn;
     compcell cc;
     lsys ps;
+    lsys us[DIM]; /* semi-implicit velocity systems (slvu[dim]) */
     int pin;
 };
 
@@ -696,6 +697,56 @@ static void explicit_euler_intermediate_velocity(hfo_solver *s, bool ve)
     }
 }
 
+/* higflow/src/hig-flow-step.c:789-885 and hig-flow-step-viscoelastic.c:1108-1209:
+ * explicit convection/pressure/source (and tensor) terms, implicit diffusion
+ * (I - dt/Re L) u* = rhs, one system per velocity component. */
+static void semi_implicit_euler_intermediate_velocity(hfo_solver *s, bool ve)
+{
+    for (int dim = 0; dim < DIM; dim++) {
+        hfacetdomain *f = s->sfdu[dim];
+        hstencil *stn = &s->stn;
+        for (int fl = 0; fl < f->nfacets; fl++) {
+            Point fc, fd;
+            hfo_facet_center(&f->facets[fl], fc);
+            hfo_delta(f->facets[fl].c, fd);
+            real rhs = 0.0;
+            if (!ve) {
+                computational_cell(s, fl, fc, fd, dim, s->u);
+                rhs += source_term(s);
+                rhs -= pressure_term(s);
+                rhs -= convective_term(s, fd);
+            } else {
+                computational_cell_ve(s, fl, fc, fd, dim, s->u);
+                rhs += source_term(s);
+                rhs -= pressure_term(s);
+                rhs += tensor_term(s);
+                rhs -= convective_term(s, fd);
+            }
+            rhs *= s->par.dt;
+            rhs += s->cc.ufacet;
+            stn_reset(stn);
+            stn->rhs = rhs;
+            real alpha = 0.0;
+            for (int dim2 = 0; dim2 < DIM; dim2++) {
+                real w = -s->par.dt / (s->par.Re * fd[dim2] * fd[dim2]);
+                alpha -= 2.0 * w;
+                Point p;
+                memcpy(p, fc, sizeof p);
+                p[dim2] = fc[dim2] + fd[dim2];
+                hfo_sfd_get_stencil(f, fc, p, w, stn);
+                p[dim2] = fc[dim2] - fd[dim2];
+                hfo_sfd_get_stencil(f, fc, p, w, stn);
+            }
+            alpha = 1.0 + alpha;
+            hfo_sfd_get_stencil(f, fc, fc, alpha, stn);
+            lsys_set_bi(&s->us[dim], fl, stn->rhs);
+            lsys_set_row(&s->us[dim], fl, stn->n, stn->ids, stn->vals);
+        }
+        lsys_solve(&s->us[dim], s->par.rtol, s->par.atol, s->par.maxit);
+        memcpy(s->ustar[dim], s->us[dim].x, sizeof(real) * (size_t)f->nfacets);
+    }
+}
+
 /* higflow/src/hig-flow-step.c:14-45 */
 static void remove_pressure_singularity(hfo_solver *s)
 {
@@ -1224,6 +1275,7 @@ void hfo_destroy(hfo_solver *s)
         free(s->dp);
         free(s->F);
         lsys_free(&s->ps);
+        for (int d = 0; d < DIM; d++) lsys_free(&s->us[d]);
     }
     for (int t = 0; t < s->ntrees; t++) hfo_destroy_tree(s->trees[t]);
     free(s->trees);
@@ -1325,6 +1377,7 @@ int hfo_setup(hfo_solver *s, const hfo_params *par)
         }
     }
     lsys_init(&s->ps, nc);
+    for (int d = 0; d < DIM; d++) lsys_init(&s->us[d], s->sfdu[d]->nfacets);
     /* hig-flow-step.c:22-28: the leaf holding the centre of tree 0 */
     {
         Point c0;
@@ -1463,8 +1516,9 @@ int hfo_bc_userid(hfo_solver *s, int i)
 int hfo_step(hfo_solver *s)
 {
     if (!s->ready) return -1;
-    if (s->par.tempdiscrtype != EXPLICIT_EULER) return -3;
-    explicit_euler_intermediate_velocity(s, false);
+    if (s->par.tempdiscrtype == EXPLICIT_EULER) explicit_euler_intermediate_velocity(s, false);
+    else if (s->par.tempdiscrtype == SEMI_IMPLICIT_EULER) semi_implicit_euler_intermediate_velocity(s, false);
+    else return -3;
     pressure(s);
     final_velocity(s);
     final_pressure(s);
@@ -1476,11 +1530,12 @@ int hfo_step(hfo_solver *s)
 int hfo_step_viscoelastic(hfo_solver *s)
 {
     if (!s->ready) return -1;
-    if (s->par.tempdiscrtype != EXPLICIT_EULER) return -3;
+    if (s->par.tempdiscrtype != EXPLICIT_EULER && s->par.tempdiscrtype != SEMI_IMPLICIT_EULER) return -3;
     velocity_derivative_tensor(s);
     explicit_constitutive(s);
     polymeric_tensor(s);
-    explicit_euler_intermediate_velocity(s, true);
+    if (s->par.tempdiscrtype == EXPLICIT_EULER) explicit_euler_intermediate_velocity(s, true);
+    else semi_implicit_euler_intermediate_velocity(s, true);
     pressure(s);
     final_velocity(s);
     final_pressure(s);

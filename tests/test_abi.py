@@ -42,7 +42,9 @@ def host_only(monkeypatch):
 
 
 @pytest.mark.parametrize("make", [lambda: cases.cavity(16), lambda: cases.newt_channel(nx=40, ny=10),
-                                  lambda: cases.cavity(4, dim=3)])
+                                  lambda: cases.cavity(4, dim=3),
+                                  lambda: cases.contraction(levels=2, h0=0.5, length=8.0),
+                                  lambda: cases.newt_channel(nx=40, ny=10, tempdiscrtype=3)])
 def test_host_stencils_match_oracle(host_only, make):
     case = make()
     o = hfo.Oracle(case)
@@ -70,3 +72,22 @@ def test_host_stencils_match_oracle(host_only, make):
                     assert abs(ra - rb) < 1e-14
                     nq += 1
     assert nq > 100
+
+
+def test_every_binding_has_argtypes():
+    # a ctypes call without argtypes truncates pointers to 32 bits
+    L = native.load()
+    for s in declared_symbols():
+        fn = getattr(L, s)
+        assert fn.argtypes is not None, s
+
+
+def test_host_only_calls(host_only):
+    h = native.Flow(cases.cavity(8))
+    h.set_timing(True)
+    h.set_timing(False)
+    h.reset_timing()
+    assert set(h.stage_bytes()) == set(native.STAGES)
+    assert h.probe(0, 1) is None  # no device: refused, not crashed
+    with pytest.raises(native.HipError):
+        h.step()

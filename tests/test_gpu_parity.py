@@ -104,3 +104,36 @@ def test_mixed_precision_error_bound():
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
     a, b = hmx.get(1), h64.get(1)
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+
+
+def test_semi_implicit_cavity():
+    compare(cases.cavity(32, Re=10.0, dt=2e-3, tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TIGHT), 5)
+
+
+def test_semi_implicit_channel_fixture():
+    # the reference fixture was written by semi-implicit Euler
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "newt_poiseuille.npz"))
+
+    def init(sim):
+        for name, f, c in (("u0", 0, 0), ("u1", 0, 1), ("p", 1, 0)):
+            sim.set(f, c, fx[name][:, 2])
+    compare(cases.newt_channel(tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TIGHT), 2, init=init)
+
+
+def test_amr_contraction_newtonian():
+    case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, viscoelastic=False, Re=1.0,
+                             dt=1e-3, **TIGHT)
+    compare(case, 3)
+
+
+def test_amr_contraction_oldroyd_b():
+    case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, Re=0.01, Wi=1.0, beta=1.0 / 9.0,
+                             dt=1e-4, **TIGHT)
+    compare(case, 3)
+
+
+def test_ve_cavity_semi_implicit_low_re():
+    case = cases.ve_cavity(24, Re=1.0, Wi=0.5, beta=0.5, dt=5e-4, tempdiscrtype=cases.SEMI_IMPLICIT_EULER,
+                           **TIGHT)
+    compare(case, 4)
