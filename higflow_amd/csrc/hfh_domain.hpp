@@ -3,6 +3,8 @@
 // pdomain.c:1146-1221, with boundary values kept symbolic.
 #pragma once
 #include <array>
+#include <atomic>
+#include <thread>
 
 #include "hfh_mesh.hpp"
 
@@ -32,7 +34,30 @@ struct CachedWls {
     std::vector<int> ids;     // numpts ids then numbcpts boundary-local ids
     std::vector<double> w;
     std::vector<int> bcidx;   // Dirichlet-list index of each boundary sample
+    // request of a deferred computation (first call's point and search size)
+    bool pending = false;
+    double x[3] = {0, 0, 0};
+    double delta = 0;
+    bool use_dir = false;
 };
+
+// Per-thread WLS work space.
+struct Scratch {
+    std::vector<WItem> items, tmp;
+    double max_dist = 0;
+    std::unique_ptr<Wls> wls, bcwls;
+};
+
+// Threads for the setup's WLS computations: HFH_SETUP_THREADS, else the
+// hardware threads (at most 16).
+inline int setup_threads() {
+    if (const char *e = std::getenv("HFH_SETUP_THREADS")) {
+        int n = std::atoi(e);
+        if (n >= 1) return n;
+    }
+    unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, h));
+}
 
 template <int D>
 class Domain {
@@ -216,20 +241,22 @@ class Domain {
 
   private:
     int order_, maxpts_, bc_maxpts_;
-    std::unique_ptr<Wls> wls_, bcwls_;
-    std::unordered_map<std::array<int64_t, 3>, std::pair<std::shared_ptr<CachedWls>, std::shared_ptr<CachedWls>>,
-                       KeyHash>
-        cache_;  // (with_bc, without_bc)
-    std::vector<WItem> items_;
-    double max_dist_ = 0;
+    struct Entry {
+        std::shared_ptr<CachedWls> with_bc, without_bc;
+        bool undecided = false;  // without_bc follows with_bc once its samples are known
+    };
+    std::unordered_map<std::array<int64_t, 3>, Entry, KeyHash> cache_;
+    Scratch sc_;
+    bool defer_ = false;
+    std::vector<std::shared_ptr<CachedWls>> pending_;
 
-    Wls &wls() {
-        if (!wls_) wls_.reset(new Wls(D, order_, maxpts_));
-        return *wls_;
+    Wls &wls(Scratch &sc) const {
+        if (!sc.wls) sc.wls.reset(new Wls(D, order_, maxpts_));
+        return *sc.wls;
     }
-    Wls &bcwls() {
-        if (!bcwls_) bcwls_.reset(new Wls(D - 1, 1, bc_maxpts_));
-        return *bcwls_;
+    Wls &bcwls(Scratch &sc) const {
+        if (!sc.bcwls) sc.bcwls.reset(new Wls(D - 1, 1, bc_maxpts_));
+        return *sc.bcwls;
     }
 
     Vec<D> block_facet_center(const Tree<D> &tr, int b, int dir) const {
@@ -244,24 +271,24 @@ class Domain {
         return false;
     }
 
-    WItem *new_item(int maxpts, double dist) {  // domain.c:388-412
-        if (dist >= max_dist_) {
-            if ((int)items_.size() >= maxpts) return nullptr;
-            max_dist_ = dist;
+    static WItem *new_item(Scratch &sc, int maxpts, double dist) {  // domain.c:388-412
+        if (dist >= sc.max_dist) {
+            if ((int)sc.items.size() >= maxpts) return nullptr;
+            sc.max_dist = dist;
         }
-        items_.emplace_back();
-        items_.back().dist = dist;
-        return &items_.back();
+        sc.items.emplace_back();
+        sc.items.back().dist = dist;
+        return &sc.items.back();
     }
 
-    void search_tree_cells(const Tree<D> &t, const IdMap &m, const Vec<D> &x, const Vec<D> &lo,
-                           const Vec<D> &hi, int maxpts, int bcidx, unsigned char type) {
+    static void search_tree_cells(Scratch &sc, const Tree<D> &t, const IdMap &m, const Vec<D> &x,
+                                  const Vec<D> &lo, const Vec<D> &hi, int maxpts, int bcidx, unsigned char type) {
         t.bbox_cells(lo, hi, [&](int c) {
             int id = m.lookup(t.n[c].cid);
             if (id < 0) return;
             Vec<D> cc = t.center(c);
             double dist = distance<D>(x, cc);
-            WItem *it = new_item(maxpts, dist);
+            WItem *it = new_item(sc, maxpts, dist);
             if (it) {
                 it->id = id;
                 it->bc = bcidx;
@@ -271,9 +298,9 @@ class Domain {
         });
     }
 
-    void search_box(const Vec<D> &x, const Vec<D> &lo, const Vec<D> &hi) {
+    void search_box(Scratch &sc, const Vec<D> &x, const Vec<D> &lo, const Vec<D> &hi) const {
         if (fdim < 0) {
-            for (auto *t : trees) search_tree_cells(*t, map, x, lo, hi, maxpts_, -1, 0);
+            for (auto *t : trees) search_tree_cells(sc, *t, map, x, lo, hi, maxpts_, -1, 0);
             return;
         }
         for (auto *t : trees) {
@@ -282,7 +309,7 @@ class Domain {
                 if (id < 0) return;
                 Vec<D> fc = t->facet_center(f);
                 double dist = distance<D>(x, fc);
-                WItem *it = new_item(maxpts_, dist);
+                WItem *it = new_item(sc, maxpts_, dist);
                 if (it) {
                     it->id = id;
                     it->bc = -1;
@@ -339,28 +366,29 @@ class Domain {
             out.emplace_back(b->base + b->map.lookup(b->tree->n[bcell].cid), 1.0);
             return out;
         }
-        items_.clear();
-        max_dist_ = 0.0;
+        Scratch &sc = sc_;
+        sc.items.clear();
+        sc.max_dist = 0.0;
         Vec<D> lo, hi;
         for (int i = 0; i < D; i++) {
             lo[i] = boxc[i] - 3.0 * bd;
             hi[i] = boxc[i] + 3.0 * bd;
         }
-        search_tree_cells(*b->tree, b->map, px, lo, hi, bc_maxpts_, -1, 2);
-        sort_items();
-        if ((int)items_.size() > bc_maxpts_) items_.resize(bc_maxpts_);
-        for (auto &it : items_)
+        search_tree_cells(sc, *b->tree, b->map, px, lo, hi, bc_maxpts_, -1, 2);
+        sort_items(sc);
+        if ((int)sc.items.size() > bc_maxpts_) sc.items.resize(bc_maxpts_);
+        for (auto &it : sc.items)
             for (int d = b->pdir + 1; d < D; d++) it.x[d - 1] = it.x[d];
-        std::vector<double> w(items_.size() + 1);
-        bcwls().weights((int)items_.size(), items_.data(), px.v, w.data());
-        for (size_t i = 0; i < items_.size(); i++) out.emplace_back(b->base + items_[i].id, w[i]);
+        std::vector<double> w(sc.items.size() + 1);
+        bcwls(sc).weights((int)sc.items.size(), sc.items.data(), px.v, w.data());
+        for (size_t i = 0; i < sc.items.size(); i++) out.emplace_back(b->base + sc.items[i].id, w[i]);
         return out;
     }
 
-    void sort_items() {
-        if (items_.size() <= 1) return;
-        std::vector<WItem> tmp(items_.size());
-        merge_sort_items(items_.data(), (int)items_.size(), tmp.data());
+    static void sort_items(Scratch &sc) {
+        if (sc.items.size() <= 1) return;
+        sc.tmp.resize(sc.items.size());
+        merge_sort_items(sc.items.data(), (int)sc.items.size(), sc.tmp.data());
     }
 
     struct Proj {
@@ -501,11 +529,65 @@ class Domain {
         return true;
     }
 
+    // domain.c:527-599: samples in the box x +- 10 delta (unknowns, then the
+    // Dirichlet boundary centres), pruned, sorted, clipped, WLS weights; the
+    // cached form drops |w| < EPSMACH (domain.c:619-648).  Reads only the
+    // domain geometry, so deferred requests run on worker threads.
+    void compute_wls(CachedWls &cw, Scratch &sc) const {
+        Vec<D> x;
+        for (int k = 0; k < D; k++) x[k] = cw.x[k];
+        const double delta = cw.delta;
+        sc.items.clear();
+        sc.max_dist = 0.0;
+        Vec<D> lo, hi;
+        for (int k = 0; k < D; k++) {
+            lo[k] = x[k] - 10.0 * delta;
+            hi[k] = x[k] + 10.0 * delta;
+        }
+        search_box(sc, x, lo, hi);
+        if (cw.use_dir)
+            for (size_t i = 0; i < dir.size(); i++)
+                search_tree_cells(sc, *dir[i]->tree, dir[i]->map, x, lo, hi, maxpts_, (int)i, 1);
+        sort_items(sc);
+        if ((int)sc.items.size() > maxpts_) sc.items.resize(maxpts_);
+        std::vector<double> w(sc.items.size() + 1);
+        wls(sc).weights((int)sc.items.size(), sc.items.data(), x.v, w.data());
+        cw.ids.clear();
+        cw.w.clear();
+        cw.bcidx.clear();
+        cw.numpts = cw.numbcpts = 0;
+        for (size_t i = 0; i < sc.items.size(); i++)
+            if (!flt_eq(w[i], 0.0) && sc.items[i].type == 0) {
+                cw.ids.push_back(sc.items[i].id);
+                cw.w.push_back(w[i]);
+                cw.numpts++;
+            }
+        for (size_t i = 0; i < sc.items.size(); i++)
+            if (!flt_eq(w[i], 0.0) && sc.items[i].type == 1) {
+                cw.ids.push_back(sc.items[i].id);
+                cw.w.push_back(w[i]);
+                cw.bcidx.push_back(sc.items[i].bc);
+                cw.numbcpts++;
+            }
+        cw.pending = false;
+    }
+
     void interpolate(const Vec<D> &x, double delta, double alpha, Stencil &stn, bool use_dir) {
         auto key = point_key<D>(x);
-        auto found = use_cache ? cache_.find(key) : cache_.end();
+        Entry *e = nullptr;
+        if (use_cache) {
+            auto it = cache_.find(key);
+            if (it != cache_.end()) e = &it->second;
+        }
         std::shared_ptr<CachedWls> cw;
-        if (found != cache_.end()) cw = use_dir ? found->second.first : found->second.second;
+        if (e) {
+            if (!use_dir && e->undecided) {
+                if (e->with_bc->pending) compute_wls(*e->with_bc, sc_);
+                if (e->with_bc->numbcpts == 0) e->without_bc = e->with_bc;
+                e->undecided = false;
+            }
+            cw = use_dir ? e->with_bc : e->without_bc;
+        }
         if (!cw) {
             if (flt_eq(delta, 0.0)) {
                 bool got = false;
@@ -530,57 +612,68 @@ class Domain {
                     delta = std::pow(delta, 1.0 / D);
                 }
             }
-            items_.clear();
-            max_dist_ = 0.0;
-            Vec<D> lo, hi;
-            for (int k = 0; k < D; k++) {
-                lo[k] = x[k] - 10.0 * delta;
-                hi[k] = x[k] + 10.0 * delta;
-            }
-            search_box(x, lo, hi);
-            if (use_dir)
-                for (size_t i = 0; i < dir.size(); i++)
-                    search_tree_cells(*dir[i]->tree, dir[i]->map, x, lo, hi, maxpts_, (int)i, 1);
-            sort_items();
-            if ((int)items_.size() > maxpts_) items_.resize(maxpts_);
-            std::vector<double> w(items_.size() + 1);
-            wls().weights((int)items_.size(), items_.data(), x.v, w.data());
-            // cached form drops |w| < EPSMACH (domain.c:619-648); rows are
-            // built once and reused every step, so the cached form is used
             cw = std::make_shared<CachedWls>();
-            for (size_t i = 0; i < items_.size(); i++)
-                if (!flt_eq(w[i], 0.0) && items_[i].type == 0) {
-                    cw->ids.push_back(items_[i].id);
-                    cw->w.push_back(w[i]);
-                    cw->numpts++;
-                }
-            for (size_t i = 0; i < items_.size(); i++)
-                if (!flt_eq(w[i], 0.0) && items_[i].type == 1) {
-                    cw->ids.push_back(items_[i].id);
-                    cw->w.push_back(w[i]);
-                    cw->bcidx.push_back(items_[i].bc);
-                    cw->numbcpts++;
-                }
+            for (int k = 0; k < D; k++) cw->x[k] = x[k];
+            cw->delta = delta;
+            cw->use_dir = use_dir;
+            if (defer_ && use_cache) {
+                cw->pending = true;
+                pending_.push_back(cw);
+            } else {
+                compute_wls(*cw, sc_);
+            }
             if (use_cache) {
-                auto &e = cache_[key];
-                if (found != cache_.end()) {
-                    if (use_dir) e.first = cw;
-                    else e.second = cw;
+                Entry &n = cache_[key];
+                if (e) {
+                    if (use_dir) n.with_bc = cw;
+                    else n.without_bc = cw;
                 } else if (use_dir) {
-                    e.first = cw;
-                    e.second = cw->numbcpts == 0 ? cw : nullptr;
+                    n.with_bc = cw;
+                    if (cw->pending) n.undecided = true;
+                    else n.without_bc = cw->numbcpts == 0 ? cw : nullptr;
                 } else {
-                    e.first = nullptr;
-                    e.second = cw;
+                    n.without_bc = cw;
                 }
             }
         }
+        if (cw->pending) return;  // first pass of a deferred build: contents follow
         for (int i = 0; i < cw->numpts; i++) stn.add(cw->ids[i], alpha * cw->w[i]);
         for (int i = 0; i < cw->numbcpts; i++) {
             Boundary<D> *b = dir[cw->bcidx[i]];
             stn.add_rhs_bc(b->base + cw->ids[cw->numpts + i], -alpha * cw->w[cw->numpts + i]);
         }
     }
+
+  public:
+    // Deferred build: while set, WLS requests are only recorded (first call
+    // wins, as with immediate computation); resolve_pending() computes them on
+    // `nthreads` threads and settles the with/without-boundary sharing.
+    void begin_deferred() { defer_ = true; }
+    void resolve_pending(int nthreads) {
+        std::vector<CachedWls *> work;
+        for (auto &p : pending_)
+            if (p->pending) work.push_back(p.get());
+        std::atomic<size_t> next{0};
+        auto run = [&]() {
+            Scratch sc;
+            for (size_t i; (i = next.fetch_add(1)) < work.size();) compute_wls(*work[i], sc);
+        };
+        int nt = (int)std::min<size_t>((size_t)std::max(1, nthreads), work.size());
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(run);
+        run();
+        for (auto &t : th) t.join();
+        for (auto &kv : cache_) {
+            Entry &e = kv.second;
+            if (e.undecided) {
+                if (e.with_bc->numbcpts == 0) e.without_bc = e.with_bc;
+                e.undecided = false;
+            }
+        }
+        pending_.clear();
+        defer_ = false;
+    }
+    size_t cache_size() const { return cache_.size(); }
 
   public:
     // domain.c:1321-1379

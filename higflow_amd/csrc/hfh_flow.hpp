@@ -230,9 +230,7 @@ class Flow : public FlowBase {
         Stencil s = stencil_at(dom, c, x);
         if (!force_row && is_identity(s)) return field + s.ids[0];
         auto k = point_key<D>(x);
-        char key[96];
-        std::snprintf(key, sizeof key, "%p:%d:%lld:%lld:%lld", (void *)&dom, field, (long long)k[0],
-                      (long long)k[1], (long long)k[2]);
+        DedupKey key{(const void *)&dom, field, {k[0], k[1], k[2]}};
         auto it = g.dedup.find(key);
         if (it != g.dedup.end()) return -(1 + it->second);
         Combo cb;
@@ -244,6 +242,7 @@ class Flow : public FlowBase {
     int combo_row(Group &g, const Combo &cb, int out) { return g.add_row(cb.cols, cb.rhs0, cb.bc, out); }
 
     void record();
+    void reset_records();
     void record_momentum(bool ve);
     void record_pressure();
     void record_projection();
@@ -385,10 +384,48 @@ int Flow<D>::setup(const Params &p) {
     o += D * D * ne;
     nX_ = o;  // P regions appended by finalize_layout
     ready_ = true;
+    int nth = setup_threads();
+    if (nth > 1) {
+        // pass 1 records every WLS request, the requests are computed in
+        // parallel, pass 2 records the rows from the filled caches
+        std::vector<Domain<D> *> doms{sdp_.get(), sdF_.get(), sdED_.get()};
+        for (int d = 0; d < D; d++) {
+            doms.push_back(sfdu_[d].get());
+            doms.push_back(sfdF_[d].get());
+        }
+        for (auto *dm : doms) dm->begin_deferred();
+        record();
+        for (auto *dm : doms) dm->resolve_pending(nth);
+        reset_records();
+    }
     record();
     finalize_layout();
     return 0;
 }
+
+template <int D>
+void Flow<D>::reset_records() {
+    for (Group *g : {&gDu_, &gK_, &gCv_, &gMom_, &gDiv_, &gGrad_}) *g = Group();
+    for (int d = 0; d < D; d++) {
+        momSlots_[d].clear();
+        momFlags_[d].clear();
+        momDelta_[d].clear();
+        Hrp_[d].clear();
+        Hci_[d].clear();
+        Hv_[d].clear();
+        Hrhs0_[d].clear();
+        Hbc_[d].clear();
+        Hsym_[d] = false;
+    }
+    veSlots_.clear();
+    veFlags_.clear();
+    veDelta_.clear();
+    Arp_.clear();
+    Aci_.clear();
+    Av_.clear();
+    symmetric_ = true;
+}
+
 
 template <int D>
 void Flow<D>::record() {

@@ -571,6 +571,9 @@ class Wls {
                         if (k > m) return;
                     }
     }
+    // Rows i and k of Q have no entries beyond column k when qr() rotates
+    // them (rotations run i-major, k ascending), so the Q update stops there;
+    // the skipped entries are zero in both rows and stay zero.
     void rotate(int m, int n, int i, int k, double s1, double s2) {
         double s, t;
         if (std::fabs(s2) + std::fabs(s1) > 0.0) {
@@ -584,7 +587,8 @@ class Wls {
                 A(i, j) = s;
                 A(k, j) = t;
             }
-            for (int j = 0; j < m; j++) {
+            const int qe = std::min(m, std::max(i, k) + 1);
+            for (int j = 0; j < qe; j++) {
                 s = s1 * Q(i, j) + s2 * Q(k, j);
                 t = -s2 * Q(i, j) + s1 * Q(k, j);
                 Q(i, j) = s;

@@ -25,6 +25,25 @@ enum { CELL_CENTRAL = 0, CELL_CUBISTA = 1 };
 // One interpolation group: rows  out = cst + sum val*X[col]  (columns are
 // absolute state offsets of component 0; ncomp components are evaluated
 // with in_stride/out_stride).
+// identity of a point evaluation: (domain, field offset, point key)
+struct DedupKey {
+    const void *dom;
+    int field;
+    int64_t k[3];
+    bool operator==(const DedupKey &o) const {
+        return dom == o.dom && field == o.field && k[0] == o.k[0] && k[1] == o.k[1] && k[2] == o.k[2];
+    }
+};
+struct DedupHash {
+    size_t operator()(const DedupKey &x) const {
+        uint64_t h = (uint64_t)(uintptr_t)x.dom * 0x9E3779B97F4A7C15ull ^ (uint64_t)x.field;
+        for (int d = 0; d < 3; d++) {
+            h ^= (uint64_t)x.k[d] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        }
+        return (size_t)h;
+    }
+};
+
 struct Group {
     std::vector<int> rowptr{0};
     std::vector<int> col;
@@ -33,7 +52,7 @@ struct Group {
     std::vector<std::vector<std::pair<int, double>>> bc;  // (bc slot, rhs coefficient)
     std::vector<int> out;  // destination offset (state, or external vector index)
     int ncomp = 1, in_stride = 0;
-    std::unordered_map<std::string, int> dedup;
+    std::unordered_map<DedupKey, int, DedupHash> dedup;
 
     int nrows() const { return (int)out.size(); }
     int add_row(const std::vector<std::pair<int, double>> &cols, double rhs0v,
