@@ -25,23 +25,32 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
-# workloads built today (see DESIGN.md for the configuration map)
+# Workloads.  The default is the 2D Oldroyd-B planar 4:1 contraction on a
+# 3-level HiG-Tree AMR mesh (the configuration the metric is quoted on); the
+# cavities are smaller and faster cases of the same step.
 WORKLOADS = {
+    # channel of length 20, half-width 4 -> 1, base cells 0.1, 3 factor-2
+    # levels within 1.0 of the re-entrant corners (47,800 leaves), parabolic
+    # inflow of mean 1, Re=0.01, Wi=1, beta=1/9, log-conformation from the
+    # identity, semi-implicit Euler, dt=1e-4, 200 steps
+    "contraction2d_3lev_oldroydb": dict(kind="contraction", levels=3, steps=200),
     # 2D Newtonian lid-driven cavity, uniform 64x64, Re=100, dt=1e-3,
-    # explicit Euler, incremental projection, CUBISTA, Poisson 1e-10
-    "cavity2d_64_re100": dict(kind="cavity", n=64, Re=100.0, dt=1e-3),
-    # the same cavity on a 512x512 mesh (dt limited by the explicit viscous bound)
-    "cavity2d_256_re100": dict(kind="cavity", n=256, Re=100.0, dt=1e-4),
+    # explicit Euler, incremental projection, CUBISTA, 100 steps
+    "cavity2d_64_re100": dict(kind="cavity", n=64, Re=100.0, dt=1e-3, steps=100),
+    # the same cavity on 256x256 (dt within the explicit viscous bound)
+    "cavity2d_256_re100": dict(kind="cavity", n=256, Re=100.0, dt=1e-4, steps=100),
     # 2D Oldroyd-B cavity (log-conformation), uniform 128x128
-    "ve_cavity2d_128": dict(kind="ve_cavity", n=128, Re=1.0, Wi=1.0, beta=0.5, dt=1e-5),
+    "ve_cavity2d_128": dict(kind="ve_cavity", n=128, Re=1.0, Wi=1.0, beta=0.5, dt=1e-5, steps=100),
 }
-DEFAULT = "cavity2d_64_re100"
+DEFAULT = "contraction2d_3lev_oldroydb"
 
 
 def make_case(spec, **tol):
     from higflow_amd import cases
     if spec["kind"] == "cavity":
         return cases.cavity(spec["n"], Re=spec["Re"], dt=spec["dt"], **tol)
+    if spec["kind"] == "contraction":
+        return cases.contraction(levels=spec["levels"], **tol)
     return cases.ve_cavity(spec["n"], Re=spec["Re"], Wi=spec["Wi"], beta=spec["beta"], dt=spec["dt"], **tol)
 
 
@@ -73,7 +82,9 @@ def probe(sim, which, reps):
     return sim.probe(which, reps)
 
 
-def cpu_baseline(case, budget_s=15.0, max_steps=400):
+def cpu_baseline(case, ncells, budget_s=15.0, max_steps=400):
+    """The CPU oracle (FP64, BiCGStab + ILU(0), one core) on the first steps
+    of the same workload, until `budget_s` seconds of step time have run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import hfo
     o = hfo.Oracle(case)
@@ -83,13 +94,13 @@ def cpu_baseline(case, budget_s=15.0, max_steps=400):
         o.step()
         n += 1
     wall = time.perf_counter() - t0
-    return case["ncells"] * n / wall, n, wall
+    return ncells * n / wall, n, wall
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
@@ -99,16 +110,25 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     spec = WORKLOADS[args.workload]
+    if args.steps is None:
+        args.steps = spec["steps"]
     ref_tol = dict(rtol=1e-10, atol=1e-10, maxit=50000)
     case = make_case(spec, **ref_tol)
-    ncells = case["ncells"]
 
     sim64, wall64 = run_gpu(case, args.steps, args.warmup, precision=0)
+    ncells = sim64.size(1)
+    print("bench: %s, %d cells, FP64 %d steps in %.3f s" % (args.workload, ncells, args.steps, wall64),
+          file=sys.stderr, flush=True)
     simmx, wallmx = run_gpu(case, args.steps, args.warmup, precision=1, inner_rtol=1e-5)
     cups64 = ncells * args.steps / wall64
     cupsmx = ncells * args.steps / wallmx
+    print("bench: mixed %d steps in %.3f s" % (args.steps, wallmx), file=sys.stderr, flush=True)
     errs = {}
-    for name, f, c in [("u%d" % d, 0, d) for d in range(case["dim"])] + [("p", 1, 0)]:
+    D = case["dim"]
+    fields = [("u%d" % d, 0, d) for d in range(D)] + [("p", 1, 0)]
+    if case["params"].get("flowtype") == 3:
+        fields += [("psi%d%d" % (i, j), 6, i * D + j) for i in range(D) for j in range(i, D)]
+    for name, f, c in fields:
         a, b = simmx.get(f, c), sim64.get(f, c)
         nb = np.linalg.norm(b)
         errs[name] = dict(l2=float(np.linalg.norm(a - b) / nb) if nb > 0 else 0.0,
@@ -133,14 +153,14 @@ def main():
                 "kernel": "k_spmv_dot<double>", "bytes_per_launch": by, "avg_us": round(ms * 1e3, 3)}
     kernels = {}
     for which, name in ((1, "k_momentum"), (2, "k_interp(momentum rows)"), (3, "k_interp(divergence rows)"),
-                        (4, "k_interp(projection rows)")):
+                        (4, "k_interp(projection rows)"), (5, "k_constitutive")):
         p = probe(sim64, which, 200)
         if p:
             kernels[name] = {"avg_us": round(p[0] * 1e3, 3), "bytes": p[1],
                              "GBps": round(p[1] / (p[0] * 1e-3) / 1e9, 2)}
     cpu = None
     if not args.no_cpu:
-        v, n, wall = cpu_baseline(case)
+        v, n, wall = cpu_baseline(case, ncells)
         cpu = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": "%d steps of %s on the CPU oracle (BiCGStab+ILU(0), rtol=atol=1e-10), %.1f s"
                          % (n, args.workload, wall)}
@@ -158,7 +178,9 @@ def main():
         "dtype": "f64-refinement/f32-inner",
         "data": "synthetic",
         "config": {"workload": args.workload, "cells": ncells, "dim": case["dim"],
-                   "dt": case["params"]["dt"], "Re": case["params"]["Re"]},
+                   "dt": case["params"]["dt"], "Re": case["params"]["Re"],
+                   "flowtype": "viscoelastic" if case["params"].get("flowtype") == 3 else "newtonian",
+                   "time_scheme": "semi-implicit" if case["params"].get("tempdiscrtype") == 3 else "explicit"},
         "fp64_value": round(cups64, 1),
         "fp64_ms_per_step": round(wall64 / args.steps * 1e3, 4),
         "mixed_over_fp64": round(cupsmx / cups64, 3),
