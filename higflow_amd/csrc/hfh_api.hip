@@ -208,7 +208,8 @@ int hfh_slv_solve(hfh_slv *s) {
         if (s->n == 0) return 0;
         s->db.upload(s->b.data(), s->n, s->stream);
         s->dx.upload(s->x.data(), s->n, s->stream);
-        s->st = s->sys.bicgstab(s->db.p, s->dx.p, s->rtol, s->atol, s->maxit, s->stream);
+        s->st = s->sys.fused ? s->sys.fused64(false, s->db.p, s->dx.p, s->rtol, s->atol, s->maxit, s->stream)
+                             : s->sys.bicgstab(s->db.p, s->dx.p, s->rtol, s->atol, s->maxit, s->stream);
         HFH_CHECK(hipMemcpyAsync(s->x.data(), s->dx.p, s->n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
         HFH_CHECK(hipStreamSynchronize(s->stream));
         return 0;
@@ -407,6 +408,14 @@ int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes) {
         int r = s->f->probe(which, reps, avg_ms, bytes);
         if (r < 0) g_err = "hfh_probe: unknown kernel id for this case, or no device";
         return r;
+    });
+}
+int hfh_operator_csr(hfh_solver *s, int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v) {
+    HFH_SOLVER(s);
+    if (!n || !nnz || !sym) return fail("null size pointers");
+    return guard([&] {
+        if (s->f->operator_csr(which, n, nnz, sym, rp, ci, v) < 0) return fail("no such operator");
+        return 0;
     });
 }
 long hfh_stage_bytes(hfh_solver *s, int stage) { return (s && s->f) ? s->f->bytes(stage) : -1; }

@@ -67,6 +67,7 @@ class FlowBase {
     virtual void set_timing(bool on) = 0;
     virtual int probe(int which, int reps, double *avg_ms, long *bytes) = 0;
     virtual void sync() = 0;
+    virtual int operator_csr(int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v) = 0;
 };
 
 template <int D>
@@ -150,6 +151,28 @@ class Flow : public FlowBase {
     void set_timing(bool on) override { timing_on_ = on; }
     int probe(int which, int reps, double *avg_ms, long *bytes) override;
     void sync() override { HFH_CHECK(hipStreamSynchronize(st_)); }
+    // the assembled linear systems: 0 pressure (pinned unknown eliminated,
+    // rows negated), 1+d semi-implicit Helmholtz system of component d
+    int operator_csr(int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v) override {
+        if (!ready_) return -1;
+        const std::vector<int> *R, *Cc;
+        const std::vector<double> *V;
+        bool sy;
+        if (which == 0) {
+            R = &Arp_, Cc = &Aci_, V = &Av_, sy = symmetric_;
+        } else if (which >= 1 && which <= D && !Hrp_[which - 1].empty()) {
+            R = &Hrp_[which - 1], Cc = &Hci_[which - 1], V = &Hv_[which - 1], sy = Hsym_[which - 1];
+        } else {
+            return -1;
+        }
+        *n = (int)R->size() - 1;
+        *nnz = (int)Cc->size();
+        *sym = sy ? 1 : 0;
+        if (rp) std::copy(R->begin(), R->end(), rp);
+        if (ci) std::copy(Cc->begin(), Cc->end(), ci);
+        if (v) std::copy(V->begin(), V->end(), v);
+        return 0;
+    }
 
   private:
     // ------------------------------------------------------------ host

@@ -347,7 +347,12 @@ int Flow<D>::step(bool ve) {
         for (int d = 0; d < D; d++) {
             if (Nf_[d] == 0) continue;
             double *x = X + oUS_[d];
-            if (par_.precision == 1)
+            if (hsys_[d].fused && par_.precision == 1)
+                hstats_[d] = hsys_[d].fused_mixed(Hsym_[d], hb_[d].p, x, par_.rtol, par_.atol, par_.inner_rtol,
+                                                  par_.maxit, st_);
+            else if (hsys_[d].fused)
+                hstats_[d] = hsys_[d].fused64(Hsym_[d], hb_[d].p, x, par_.rtol, par_.atol, par_.maxit, st_);
+            else if (par_.precision == 1)
                 hstats_[d] = hsys_[d].mixed(hb_[d].p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
             else if (Hsym_[d])
                 hstats_[d] = hsys_[d].pcg(hb_[d].p, x, par_.rtol, par_.atol, par_.maxit, st_);
@@ -361,7 +366,11 @@ int Flow<D>::step(bool ve) {
     tend(ST_DIV);
     tbegin(ST_SOLVE);
     double *x = X + (par_.projtype == INCREMENTAL ? oDP_ : oP_);
-    if (par_.precision == 1) {
+    if (psys_.fused && par_.precision == 1) {
+        stats_ = psys_.fused_mixed(symmetric_, bvec_.p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
+    } else if (psys_.fused) {
+        stats_ = psys_.fused64(symmetric_, bvec_.p, x, par_.rtol, par_.atol, par_.maxit, st_);
+    } else if (par_.precision == 1) {
         stats_ = psys_.mixed(bvec_.p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
     } else if (symmetric_) {
         stats_ = psys_.pcg(bvec_.p, x, par_.rtol, par_.atol, par_.maxit, st_);

@@ -227,6 +227,12 @@ __global__ void __launch_bounds__(BLOCK) k_copy(int n, const T *src, T *dst) {
     if (i < n) dst[i] = src[i];
 }
 
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_fill(int n, T *x, T v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
 inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace hfh

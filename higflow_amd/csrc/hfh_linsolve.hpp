@@ -16,11 +16,14 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "hfh_kernels.hpp"
+#include "hfh_krylov.hpp"
 
 #define HFH_CHECK(x)                                                                        \
     do {                                                                                    \
@@ -289,11 +292,6 @@ __global__ void __launch_bounds__(BLOCK) k_axpy_sc(int n, T *x, const T *y, cons
     if (i < n) x[i] = x[i] + sc[slot] * y[i];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_fill(int n, T *x, T v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
-}
 
 template <typename TD, typename TS>
 __global__ void __launch_bounds__(BLOCK) k_convert(int n, const TS *a, TD *b) {
@@ -373,6 +371,17 @@ class DeviceSystem {
     DevBuf<float> fpart1, fpart2, fpart3, fsc;
     int nb = 0;
     int check_every = 8;
+    // fused iterations (FP64 solve and FP32 inner solve of the mixed form)
+    FusedKrylov<double> k64;
+    FusedKrylov<float> k32;
+    bool fused = true;
+
+    DeviceSystem() = default;
+    DeviceSystem(const DeviceSystem &) = delete;
+    DeviceSystem &operator=(const DeviceSystem &) = delete;
+    ~DeviceSystem() {
+        if (hpin_) (void)hipHostFree(hpin_);
+    }
 
     void set_matrix(const std::vector<int> &hrp, const std::vector<int> &hci, const std::vector<double> &hv,
                     hipStream_t st) {
@@ -401,8 +410,97 @@ class DeviceSystem {
             if ((int)b->n < nb) b->alloc(nb);
         if (!sc.p) sc.alloc(S_NSC);
         if (!fsc.p) fsc.alloc(S_NSC);
+        if (!hpin_) HFH_CHECK(hipHostMalloc((void **)&hpin_, 512, hipHostMallocDefault));
+        k64.set(n, rp.p, ci.p, hrp, hci, hv, st);
+        k32.set(n, rp.p, ci.p, hrp, hci, hv, st);
+        // HFH_KRYLOV=classic selects the one-kernel-per-operation iterations
+        const char *e = std::getenv("HFH_KRYLOV");
+        fused = !(e && std::strcmp(e, "classic") == 0);
         HFH_CHECK(hipStreamSynchronize(st));
     }
+
+    // ------------------------------------------------- fused iterations
+    // copies the flags and scalars of one fused solve to pinned host memory
+    template <typename T>
+    void read_status(const FusedKrylov<T> &k, int *hi, double *hs, hipStream_t st) {
+        HFH_CHECK(hipMemcpyAsync(hpin_, k.ki.p, KI_N * sizeof(int), hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipMemcpyAsync(hpin_ + 64, k.ks.p, KS_N * sizeof(T), hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipStreamSynchronize(st));
+        std::memcpy(hi, hpin_, KI_N * sizeof(int));
+        const T *t = reinterpret_cast<const T *>(hpin_ + 64);
+        for (int q = 0; q < KS_N; q++) hs[q] = (double)t[q];
+    }
+
+    // runs one fused solve to its stopping test: the predicted number of
+    // iterations, then host checks every 8; returns the iterations used
+    template <typename T>
+    int run_fused(FusedKrylov<T> &k, bool cg, int maxit, T rtol, T atol, T *x, int *hi, double *hs, hipStream_t st) {
+        int it = 0;
+        int chunk = std::min(maxit + 1, std::max(k.pred + 1, 2));
+        for (;;) {
+            k.iterate(cg, it, chunk, maxit, rtol, atol, x, st);
+            it += chunk;
+            read_status<T>(k, hi, hs, st);
+            if (hi[KI_DONE] || it > maxit) break;
+            chunk = std::min(check_every, maxit + 1 - it);
+        }
+        const int used = hi[KI_DONE] ? hi[KI_ITERS] : it;
+        k.pred = used;
+        return used;
+    }
+
+    // FP64 fused solve: Jacobi-PCG (spd) or BiCGStab, warm-started from x
+    SolveStats fused64(bool spd, const double *b, double *x, double rtol, double atol, int maxit, hipStream_t st) {
+        int hi[KI_N];
+        double hs[KS_N];
+        k64.init<double>(spd, b, x, st);
+        SolveStats S;
+        S.iters = run_fused<double>(k64, spd, maxit, rtol, atol, x, hi, hs, st);
+        S.converged = hi[KI_DONE] && hi[KI_CONV];
+        S.resid = hs[KS_BN] > 0 ? std::sqrt(hs[KS_RR]) / hs[KS_BN] : std::sqrt(hs[KS_RR]);
+        return S;
+    }
+
+    // Mixed: FP64 defect correction around fused FP32 solves (PCG or
+    // BiCGStab) of A e = r from e = 0 to inner_rtol.  Each inner solve runs
+    // to its own tolerance (a truncated BiCGStab correction can raise the
+    // error).
+    SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
+                           int maxit, hipStream_t st) {
+        SolveStats S;
+        double h[S_NSC];
+        int hi[KI_N];
+        double hs[KS_N];
+        int outer = 0, inner_total = 0;
+        double bn = 0, rn = 0, tol = 0;
+        for (;;) {
+            k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
+            HFH_CHECK(hipMemcpyAsync(hpin_ + 256, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+            std::memcpy(h, hpin_ + 256, sizeof h);
+            bn = std::sqrt(h[S_BB]);
+            rn = std::sqrt(h[S_RR]);
+            tol = std::max(rtol * bn, atol);
+            if (rn <= tol || outer >= 50 || inner_total >= maxit || !std::isfinite(rn)) break;
+            k_convert<float, double><<<nb, BLOCK, 0, st>>>(n, r.p, b32.p);
+            k_fill<float><<<nb, BLOCK, 0, st>>>(n, x32.p, 0.0f);
+            k32.init<float>(spd, b32.p, x32.p, st);
+            const int used = run_fused<float>(k32, spd, maxit - inner_total, (float)inner_rtol, 0.0f, x32.p, hi, hs,
+                                              st);
+            inner_total += std::max(used, 1);
+            k_add_conv<double, float><<<nb, BLOCK, 0, st>>>(n, x, x32.p);
+            outer++;
+        }
+        S.iters = inner_total;
+        S.outer = outer;
+        S.resid = bn > 0 ? rn / bn : rn;
+        S.converged = rn <= tol;
+        return S;
+    }
+
+    int lanes64() const { return k64.lanes; }
+    int lanes32() const { return k32.lanes; }
 
     // Jacobi-PCG, FP64.  x holds the initial guess (warm start).
     SolveStats pcg(const double *b, double *x, double rtol, double atol, int maxit, hipStream_t st) {
@@ -553,6 +651,9 @@ class DeviceSystem {
         S.converged = rn <= tol;
         return S;
     }
+
+  private:
+    char *hpin_ = nullptr;  // pinned status: flags at 0, fused scalars at 64, outer scalars at 256
 };
 
 }  // namespace hfh

@@ -95,6 +95,7 @@ def load():
     L.hfh_stage_bytes.argtypes = [vp, C.c_int]
     L.hfh_stage_bytes.restype = C.c_long
     L.hfh_sync.argtypes = [vp]
+    L.hfh_operator_csr.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip, dp]
     L.hfh_set_timing.argtypes = [vp, C.c_int]
     L.hfh_probe.argtypes = [vp, C.c_int, C.c_int, dp, lp]
     L.hfh_version.argtypes = []
@@ -106,6 +107,10 @@ def load():
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def _check(L, rc, what):
@@ -246,6 +251,19 @@ class Flow:
 
     def stage_bytes(self):
         return {s: self.L.hfh_stage_bytes(self.h, i) for i, s in enumerate(STAGES)}
+
+    def operator(self, which=0):
+        """(rowptr, cols, vals, symmetric) of an assembled system: 0 pressure,
+        1+d the semi-implicit velocity system of component d."""
+        n, nnz, sym = C.c_int(), C.c_int(), C.c_int()
+        _check(self.L, self.L.hfh_operator_csr(self.h, which, C.byref(n), C.byref(nnz), C.byref(sym), None, None,
+                                               None), "hfh_operator_csr")
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        v = np.zeros(nnz.value)
+        _check(self.L, self.L.hfh_operator_csr(self.h, which, C.byref(n), C.byref(nnz), C.byref(sym), _ip(rp),
+                                               _ip(ci), _dp(v)), "hfh_operator_csr")
+        return rp, ci, v, bool(sym.value)
 
 
 class LinearSolver:

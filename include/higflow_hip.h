@@ -157,6 +157,12 @@ int hfh_set_timing(hfh_solver *s, int on);
  * launch and the algorithmic bytes of one launch. */
 int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
 int hfh_sync(hfh_solver *s);
+/* the assembled systems as CSR: which 0 pressure (the matrix slv_set_Ai
+ * receives in higflow_pressure, hig-flow-step.c:48-131, with the pinned
+ * unknown eliminated and rows negated), 1+d the semi-implicit velocity
+ * system of component d (hig-flow-step.c:789-885).  Call with null rp/ci/v
+ * for the sizes; *sym is 1 when the matrix is symmetric. */
+int hfh_operator_csr(hfh_solver *s, int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v);
 
 #ifdef __cplusplus
 }

@@ -141,3 +141,46 @@ def test_ve_cavity_semi_implicit_low_re():
     case = cases.ve_cavity(24, Re=1.0, Wi=0.5, beta=0.5, dt=5e-4, tempdiscrtype=cases.SEMI_IMPLICIT_EULER,
                            **TIGHT)
     compare(case, 4)
+
+
+def _agree(a, b, dim, ve, tol):
+    fa, fb = fields(a, dim, ve), fields(b, dim, ve)
+    uscale = max(np.abs(fb[("u", d)]).max() for d in range(dim))
+    for key in fb:
+        scale = uscale if key[0] == "u" else max(np.abs(fb[key]).max(), 1e-300)
+        assert np.abs(fa[key] - fb[key]).max() / scale <= tol, key
+
+
+@pytest.mark.parametrize("which", ["cavity", "contraction"])
+def test_fused_and_classic_krylov_agree(which, monkeypatch):
+    # fused iterations (2 launches per PCG / 3 per BiCGStab iteration, device
+    # stopping test) against one kernel per operation
+    if which == "cavity":
+        case = cases.cavity(32, **TIGHT)
+    else:
+        case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, Re=0.01, Wi=1.0, beta=1.0 / 9.0,
+                                 dt=1e-4, **TIGHT)
+    monkeypatch.setenv("HFH_KRYLOV", "classic")
+    hc = native.Flow(case)
+    monkeypatch.delenv("HFH_KRYLOV")
+    hf = native.Flow(case)
+    for _ in range(3):
+        hc.step()
+        hf.step()
+    _agree(hf, hc, case["dim"], case["params"]["flowtype"] == 3, 1e-10)
+
+
+def test_mixed_precision_amr_contraction():
+    # nonsymmetric systems: FP32 BiCGStab corrections inside FP64 refinement
+    case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, Re=0.01, Wi=1.0, beta=1.0 / 9.0,
+                             dt=1e-4, rtol=1e-10, atol=1e-10, maxit=50000)
+    h64 = native.Flow(case)
+    hmx = native.Flow(case, precision=1, inner_rtol=1e-5)
+    for _ in range(5):
+        h64.step()
+        hmx.step()
+    for d in range(2):
+        a, b = hmx.get(0, d), h64.get(0, d)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+    a, b = hmx.get(1), h64.get(1)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
