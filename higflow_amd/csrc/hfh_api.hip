@@ -374,6 +374,18 @@ int hfh_pinned_cell(hfh_solver *s) {
     HFH_SOLVER(s);
     return s->f->pinned();
 }
+int hfh_solve_stats(hfh_solver *s, int which, int *info, double *resid) {
+    HFH_SOLVER(s);
+    if (!info || !resid) return fail("null output pointers");
+    hfh::SolveStats st;
+    if (s->f->solve_stats(which, &st) < 0) return fail("no such system");
+    info[0] = st.iters;
+    info[1] = st.outer;
+    info[2] = st.fallback;
+    info[3] = st.restarts;
+    *resid = st.resid;
+    return st.converged ? 1 : 0;
+}
 int hfh_last_solve(hfh_solver *s, int *iters, double *resid, int *outer) {
     HFH_SOLVER(s);
     hfh::SolveStats st = s->f->last_solve();

@@ -61,6 +61,7 @@ class FlowBase {
     virtual double time() const = 0;
     virtual int pinned() const = 0;
     virtual SolveStats last_solve() const = 0;
+    virtual int solve_stats(int which, SolveStats *out) const = 0;
     virtual void timing(double *ms_per_stage, long *calls) = 0;
     virtual void reset_timing() = 0;
     virtual long bytes(int stage) const = 0;
@@ -135,6 +136,18 @@ class Flow : public FlowBase {
     double time() const override { return par_.t; }
     int pinned() const override { return desing_ ? pin_ : -1; }
     SolveStats last_solve() const override { return stats_; }
+    // 0 pressure, 1+d semi-implicit velocity component d
+    int solve_stats(int which, SolveStats *out) const override {
+        if (which == 0) {
+            *out = stats_;
+            return 0;
+        }
+        if (which >= 1 && which <= D) {
+            *out = hstats_[which - 1];
+            return 0;
+        }
+        return -1;
+    }
     void timing(double *ms, long *calls) override {
         for (int s = 0; s < ST_NSTAGE; s++) {
             ms[s] = ms_[s];
@@ -215,6 +228,7 @@ class Flow : public FlowBase {
         DevBuf<int> rp, col, out, y;
         DevBuf<double> val, cst;
         int nrow = 0, ncomp = 1, in_stride = 0, out_stride = 0;
+        int lanes = 1;  // lanes per row (from the mean row length)
         bool has_y = false;
     } dDu_, dK_, dCv_, dMom_, dDiv_, dGrad_;
     DevBuf<int> dMomSlots_[D], dVeSlots_;

@@ -247,6 +247,13 @@ void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool
     dg.val.upload(val.data(), val.size(), st_);
     dg.out.upload(g.out.data(), g.out.size(), st_);
     dg.cst.alloc(dg.nrow);
+    // lanes per row from the mean row length (HFH_INTERP_LANES overrides)
+    const double avg = (double)g.col.size() / dg.nrow;
+    dg.lanes = avg < 4 ? 1 : avg < 8 ? 2 : avg < 16 ? 4 : avg < 32 ? 8 : 16;
+    if (const char *e = std::getenv("HFH_INTERP_LANES")) {
+        int f = std::atoi(e);
+        if (f == 1 || f == 2 || f == 4 || f == 8 || f == 16) dg.lanes = f;
+    }
 }
 
 template <int D>
@@ -280,9 +287,22 @@ void Flow<D>::run_group(DGroup &dg, double alpha, double *dst, int stage) {
     if (dg.nrow == 0) return;
     (void)stage;
     long n = (long)dg.nrow * dg.ncomp;
-    k_interp<double><<<nblocks(n), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, dg.val.p, dg.cst.p,
-                                                    dg.out.p, dg.has_y ? dg.y.p : nullptr, alpha, dg.in_stride,
-                                                    dg.out_stride, X_.p, dst);
+    const int *y = dg.has_y ? dg.y.p : nullptr;
+#define HFH_INTERP_L(L)                                                                                      \
+    k_interp_l<double, L><<<nblocks(n * L), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, dg.val.p, \
+                                                             dg.cst.p, dg.out.p, y, alpha, dg.in_stride,     \
+                                                             dg.out_stride, X_.p, dst)
+    switch (dg.lanes) {
+    case 1:
+        k_interp<double><<<nblocks(n), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, dg.val.p, dg.cst.p,
+                                                        dg.out.p, y, alpha, dg.in_stride, dg.out_stride, X_.p, dst);
+        break;
+    case 2: HFH_INTERP_L(2); break;
+    case 4: HFH_INTERP_L(4); break;
+    case 8: HFH_INTERP_L(8); break;
+    default: HFH_INTERP_L(16); break;
+    }
+#undef HFH_INTERP_L
 }
 
 // hig-flow-step.c:1363-1413 / hig-flow-step-viscoelastic.c:2091-2163
@@ -428,6 +448,48 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
                                                             psys_.q.p, psys_.part1.p);
             by = (long)psys_.nnz * 12 + (long)(n + 1) * 4 + (long)n * 16 + (long)psys_.nb * 8;
             break;
+        case 6: {
+            // the fused pressure BiCGStab t-kernel, timed by the solver itself
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            float ms = psys_.k64.probe(reps, &by, st_);
+            *avg_ms = ms;
+            *bytes = by;
+            return 0;
+        }
+        case 7: {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            float ms = psys_.k32.probe(reps, &by, st_);
+            *avg_ms = ms;
+            *bytes = by;
+            return 0;
+        }
+        case 8: {
+            // plain copy of 8 Mi doubles (64 MiB in, 64 MiB out): the
+            // reference point that scales the hardware byte counters
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            const int nc = 8 << 20;
+            DevBuf<double> src, dst;
+            src.alloc(nc);
+            dst.alloc(nc);
+            k_fill<double><<<nblocks(nc), BLOCK, 0, st_>>>(nc, src.p, 1.0);
+            hipEvent_t c0, c1;
+            HFH_CHECK(hipEventCreate(&c0));
+            HFH_CHECK(hipEventCreate(&c1));
+            HFH_CHECK(hipEventRecord(c0, st_));
+            for (int q = 0; q < reps; q++) k_copy<double><<<nblocks(nc), BLOCK, 0, st_>>>(nc, src.p, dst.p);
+            HFH_CHECK(hipEventRecord(c1, st_));
+            HFH_CHECK(hipEventSynchronize(c1));
+            float ms = 0;
+            HFH_CHECK(hipEventElapsedTime(&ms, c0, c1));
+            (void)hipEventDestroy(c0);
+            (void)hipEventDestroy(c1);
+            *avg_ms = reps > 0 ? ms / reps : 0.0;
+            *bytes = 2L * nc * (long)sizeof(double);
+            return 0;
+        }
         case 1:
             k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[0].p, dMomFlags_[0].p,
                                                              dMomDelta_[0].p, X);

@@ -37,6 +37,52 @@ __global__ void __launch_bounds__(BLOCK) k_interp(int nrow, int ncomp, const int
     dst[out[r] + (size_t)c * out_stride] = o;
 }
 
+// The same rows with L lanes per (row, component): lane l sums entries
+// l, l+L, ...; the lane sums combine by a butterfly in fixed order.  Long
+// WLS rows (up to 45/120 entries) then run L-way parallel.
+template <typename T, int L>
+__global__ void __launch_bounds__(BLOCK) k_interp_l(int nrow, int ncomp, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col, const T *__restrict__ val,
+                                                    const T *__restrict__ cst, const int *__restrict__ out,
+                                                    const int *__restrict__ yoff, T alpha, int in_stride,
+                                                    int out_stride, const T *X, T *dst) {
+    const long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) / L;
+    const int lane = threadIdx.x % L;
+    const bool in = g < (long)nrow * ncomp;
+    const int r = in ? (int)(g % nrow) : 0, c = in ? (int)(g / nrow) : 0;
+    const int b = in ? rowptr[r] : 0, e = in ? rowptr[r + 1] : 0;
+    const T *Xc = X + (size_t)c * in_stride;
+    T v = 0;
+    // four entries per pass: indices first, then the gathered values
+    constexpr int U = 4;
+#pragma unroll 1
+    for (int k0 = b + lane; k0 < e; k0 += U * L) {
+        int j[U];
+        T a[U];
+        bool ok[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int k = k0 + q * L;
+            ok[q] = k < e;
+            j[q] = ok[q] ? col[k] : 0;
+            a[q] = ok[q] ? val[k] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const T x = Xc[j[q]];
+            if (ok[q]) v += x * a[q];
+        }
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (in && lane == 0) {
+        v = cst[r] + v;
+        T o = alpha * v;
+        if (yoff) o = X[yoff[r]] + o;
+        dst[out[r] + (size_t)c * out_stride] = o;
+    }
+}
+
 // ------------------------------------------------------------- momentum
 struct MomParams {
     int nf, ns;

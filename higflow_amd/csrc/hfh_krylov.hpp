@@ -1,28 +1,38 @@
-// Fused Krylov iterations: Jacobi-PCG in two launches per iteration and
-// right-Jacobi BiCGStab in three, with the convergence test on the device.
+// Fused Krylov iterations: Jacobi-PCG and right-Jacobi BiCGStab in two
+// launches per iteration each, with the convergence test on the device.
 //
-// * Every launch begins by summing the per-workgroup partials of the
-//   previous launch in a fixed order, redundantly in every workgroup, so each
-//   workgroup derives the same scalars (alpha, beta, omega, the stopping
-//   decision) without a reduction launch or a host round trip.
-// * SpMV operands are formed on the fly at each column from two gathered
-//   vectors: CG q = A (z + beta p_old); BiCGStab v = (A D^-1)(r + beta y) with
-//   y = p - omega v kept by the previous update, and t = (A D^-1)(r - alpha v).
+// * Every launch sums the per-workgroup partials of the previous launch in a
+//   fixed order, redundantly in every workgroup, so each workgroup derives
+//   the same scalars (alpha, beta, omega, the stopping decision) without a
+//   reduction launch or a host round trip.  One wave sums the partials
+//   while the others gather their rows; one barrier then publishes the
+//   totals (the `done` flag rides along).
+// * Gathered vectors are summed separately per row and combined once the
+//   scalars are known (CG: q = A z + beta A p_old; BiCGStab:
+//   v = (A D^-1)(s - omega t) + beta (A D^-1)(p_old - omega v_old)).
 //   The right Jacobi scaling is folded into the matrix values (A D^-1).
+// * BiCGStab takes two launches: launch A applies the previous iteration's
+//   update (x, r = s - omega t) with omega from the t.s, t.t partials and
+//   rho = rh.s - omega rh.t, then forms p and v = (A D^-1) p; launch B forms
+//   s = r - alpha v and t = (A D^-1) s.  The stopping test runs in launch B
+//   on the exact r.r of the residual launch A wrote.
 // * Each row is processed by L lanes (L = 1, 2, 4, 8, picked per matrix by
-//   timing at set-up); workgroups of 1024 threads hold 1024/L rows, so the
-//   partial count stays small while enough waves are in flight to hide the
-//   gather latency of short rows.
+//   timing at set-up); workgroups of 1024 threads hold 1024/L rows.
 //
 // Stopping: ||r|| <= max(rtol ||b||, atol) on the recursively updated
-// residual, a non-finite residual, breakdown, or `maxit` iterations.  The
-// launch that sees the test pass sets `done`; later launches return at once,
-// so the host enqueues a predicted number of iterations and checks once.
+// residual, a non-finite residual, breakdown, divergence, or `maxit`
+// iterations.  The launch that sees the test pass sets `done`; later
+// launches return at once, so the host enqueues a predicted number of
+// iterations and checks once.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <stdexcept>
+#include <vector>
 
 #include "hfh_kernels.hpp"
 
@@ -32,78 +42,94 @@ constexpr int KNT = 1024;  // threads per workgroup of the Krylov kernels
 constexpr int KNW = KNT / 64;
 
 // scalar slots (ks, type T) and integer slots (ki)
-enum { KS_TOL = 0, KS_BN, KS_RR, KS_RHO, KS_ALPHA = KS_RHO + 2, KS_OMEGA = KS_ALPHA + 2, KS_N = KS_OMEGA + 2 };
-enum { KI_DONE = 0, KI_ITERS, KI_CONV, KI_N };
+enum {
+    KS_TOL = 0,
+    KS_BN,
+    KS_RR,   // r.r when the solve stopped
+    KS_RR0,  // max(r.r, b.b) at the start (divergence scale)
+    KS_RHO,
+    KS_ALPHA = KS_RHO + 2,
+    KS_N = KS_ALPHA + 2
+};
+// KI_DONE is set by a launch that stops without updating anything after the
+// decision (every workgroup takes the same decision, so one that starts late
+// and reads the flag already set skips nothing it should have done).
+// Launch A of BiCGStab updates x and r before it can detect breakdown, so it
+// reports breakdown in KI_BRK + (it & 1) and reads only the slot of the
+// previous iteration: no workgroup skips its update on a flag its own launch
+// set.
+enum { KI_DONE = 0, KI_ITERS, KI_CONV, KI_BRK, KI_N = KI_BRK + 2 };
 // partial-sum slots (each nb long)
-enum { KP_RZ = 0, KP_RR, KP_BB, KP_PQ, KP_RV, KP_TS, KP_TT, KP_N };
+enum { KP_RZ = 0, KP_RR, KP_BB, KP_PQ, KP_RV, KP_TS, KP_TT, KP_RS, KP_RT, KP_N };
 
-// Sums K per-thread values over the workgroup; every thread gets the totals
-// (fixed order: wave tree, then the wave sums in sequence).
+// Wave 0 sums K partial arrays (nb entries each, fixed order: lane strides,
+// then a butterfly) plus the done flag (slot K) and leaves the totals in
+// `sh`; the other waves go on with their row gathers meanwhile.  Every
+// workgroup computes the same totals.  Read them after kblock_sync().
 template <typename T, int K>
-__device__ inline void kblock_allsum(T (&v)[K]) {
-    __shared__ T sh[K][KNW];
+__device__ inline void reduce_partials(const T *part, int nb, const int (&slot)[K], const int *ki, T *sh,
+                                       int brk = -1) {
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        T a = 0;
+#pragma unroll 1
+        for (int i = lane; i < nb; i += 64) a += part[(size_t)slot[k] * nb + i];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if (lane == 0) sh[k] = a;
+    }
+    if (lane == 0) sh[K] = (T)(ki[KI_DONE] + (brk >= 0 ? ki[brk] : 0));
+}
+
+__device__ inline void kblock_sync() { __syncthreads(); }
+
+// Sums K per-thread values over the workgroup and writes the totals to
+// part[slot[k] * nb + blockIdx.x] (fixed order: wave tree, then the wave
+// sums in sequence by thread 0).  Last shared-memory use of a launch.
+template <typename T, int K>
+__device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&slot)[K]) {
+    __shared__ T sw[K][KNW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         T a = v[k];
         for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-        if (lane == 0) sh[k][w] = a;
+        if (lane == 0) sw[k][w] = a;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        T s = 0;
-        for (int j = 0; j < KNW; j++) s += sh[k][j];
-        v[k] = s;
-    }
-    __syncthreads();
-}
-
-// totals of K partial arrays (slots `slot[k]` of `part`, nb entries each)
-template <typename T, int K>
-__device__ inline void sum_partials(const T *part, int nb, const int (&slot)[K], T (&out)[K]) {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
+    if (threadIdx.x < K) {
+        const int k = threadIdx.x;
         T a = 0;
-        for (int i = threadIdx.x; i < nb; i += KNT) a += part[(size_t)slot[k] * nb + i];
-        out[k] = a;
+        for (int j = 0; j < KNW; j++) a += sw[k][j];
+        part[(size_t)slot[k] * nb + blockIdx.x] = a;
     }
-    kblock_allsum<T, K>(out);
 }
 
-template <typename T, int K>
-__device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&slot)[K]) {
-    kblock_allsum<T, K>(v);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < K; k++) part[(size_t)slot[k] * nb + blockIdx.x] = v[k];
-}
-
-// the workgroup's view of the done flag, uniform over the workgroup
-__device__ inline bool krylov_done(const int *ki) {
-    __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = ki[KI_DONE];
-    __syncthreads();
-    return s_done != 0;
-}
-
-// stopping test of the first launch of an iteration; true = stop
+// stopping test; `first` computes the tolerance from ||b||.  Besides
+// convergence: a non-finite residual, breakdown, `maxit`, or divergence
+// (||r|| above 1e5 max(||r0||, ||b||), PETSc's default dtol).
 template <typename T>
-__device__ inline bool krylov_stop(int it, int maxit, T rr, T bb, T rtol, T atol, bool breakdown, T *ks, int *ki) {
-    T tol;
-    if (it == 0) {
+__device__ inline bool krylov_stop(bool first, int it, int maxit, T rr, T bb, T rtol, T atol, bool breakdown, T *ks,
+                                   int *ki) {
+    T tol, rr0;
+    if (first) {
         tol = rtol * sqrt(bb);
         if (tol < atol) tol = atol;
+        rr0 = rr > bb ? rr : bb;
     } else {
         tol = ks[KS_TOL];
+        rr0 = ks[KS_RR0];
     }
     const bool finite = isfinite(rr);
     const bool conv = finite && sqrt(rr) <= tol;
-    const bool stop = conv || !finite || breakdown || it >= maxit;
+    const bool diverged = rr > T(1e10) * rr0;
+    const bool stop = conv || !finite || breakdown || diverged || it >= maxit;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (it == 0) {
+        if (first) {
             ks[KS_TOL] = tol;
             ks[KS_BN] = sqrt(bb);
+            ks[KS_RR0] = rr0;
         }
         if (stop) {
             ks[KS_RR] = rr;
@@ -115,7 +141,7 @@ __device__ inline bool krylov_stop(int it, int maxit, T rr, T bb, T rtol, T atol
     return stop;
 }
 
-// row `i` of a workgroup of 1024/L rows, lane within the row
+// row of this thread (workgroups of 1024/L rows) and its lane
 template <int L>
 __device__ inline int krow() {
     return blockIdx.x * (KNT / L) + (int)threadIdx.x / L;
@@ -125,19 +151,45 @@ __device__ inline int klane() {
     return (int)threadIdx.x % L;
 }
 
-// sum_k val[k] * f(ci[k]) over the row's entries, split over L lanes and
-// combined so that every lane holds the row total (fixed order)
-template <typename T, int L, typename F>
-__device__ inline T row_sum(int b, int e, const int *__restrict__ ci, const T *__restrict__ val, F f) {
-    T s = 0;
-    for (int k = b + klane<L>(); k < e; k += L) s += val[k] * f(ci[k]);
+// sums over the row's entries of val[k] * vec[m][ci[k]], split over L lanes
+// and combined so that every lane holds the row totals (fixed order).  Each
+// lane takes its entries four at a time: the four column indices are loaded
+// together, then all their vector values, so a long (WLS) row costs a
+// quarter of the dependent load round trips.
+template <typename T, int L, int M>
+__device__ inline void row_sums(int rb, int re, const int *__restrict__ ci, const T *__restrict__ val,
+                                const T *const (&vec)[M], T (&out)[M]) {
+    constexpr int U = 4;
 #pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    return s;
+    for (int m = 0; m < M; m++) out[m] = 0;
+#pragma unroll 1
+    for (int k0 = rb + klane<L>(); k0 < re; k0 += U * L) {
+        int j[U];
+        T a[U];
+        bool ok[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int k = k0 + q * L;
+            ok[q] = k < re;
+            j[q] = ok[q] ? ci[k] : 0;
+            a[q] = ok[q] ? val[k] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++)
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const T x = vec[m][j[q]];
+                if (ok[q]) out[m] += a[q] * x;
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++)
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) out[m] += __shfl_xor(out[m], o, 64);
 }
 
 // ------------------------------------------------------------------ init
-// r = b - A x; CG: z = dinv r, partials r.z; BiCGStab: rh = r, partial r.r.
+// r = b - A x; CG: z = dinv r, partial r.z; BiCGStab: rh = r, partial r.r.
 // Partials r.r and b.b; clears the flags.
 template <typename T, typename TB, int L>
 __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -148,11 +200,13 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, const int *__restr
     const int i = krow<L>();
     const bool in = i < n;
     const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
-    T s = row_sum<T, L>(rb, re, ci, val, [&](int j) { return x[j]; });
+    const T *const vx[1] = {x};
+    T sx[1];
+    row_sums<T, L, 1>(rb, re, ci, val, vx, sx);
     T acc[3] = {0, 0, 0};
     if (in && klane<L>() == 0) {
         T bi = (T)b[i];
-        T ri = bi - s;
+        T ri = bi - sx[0];
         r[i] = ri;
         if (cg) {
             T zi = dinv[i] * ri;
@@ -171,44 +225,47 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, const int *__restr
         ki[KI_DONE] = 0;
         ki[KI_ITERS] = 0;
         ki[KI_CONV] = 0;
+        ki[KI_BRK] = 0;
+        ki[KI_BRK + 1] = 0;
     }
 }
 
 // ------------------------------------------------------------------- PCG
 // launch 1: stopping test, beta = rho / rho_old, p = z + beta p_old (own rows
-// to pnew), q = A p with p formed at each column; partial p.q
+// to pnew), q = A z + beta A p_old; partial p.q
 template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rtol, T atol,
                                                   const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const T *__restrict__ val, const T *__restrict__ z,
                                                   const T *__restrict__ pold, T *__restrict__ pnew,
                                                   T *__restrict__ q, T *__restrict__ part, int nb, T *ks, int *ki) {
-    if (krylov_done(ki)) return;
-    T s3[3];
     const int sl[3] = {KP_RZ, KP_RR, KP_BB};
-    sum_partials<T, 3>(part, nb, sl, s3);
-    const T rho = s3[0];
-    if (krylov_stop<T>(it, maxit, s3[1], s3[2], rtol, atol, false, ks, ki)) return;
+    __shared__ T pv[4];
+    reduce_partials<T, 3>(part, nb, sl, ki, pv);
+    const int i = krow<L>();
+    const bool in = i < n;
+    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
+    const T *const vz[2] = {z, it == 0 ? z : pold};
+    T sz[2];
+    row_sums<T, L, 2>(rb, re, ci, val, vz, sz);
+    const T zi = in ? z[i] : T(0), poi = (in && it > 0) ? pold[i] : T(0);
+    kblock_sync();
+    if (pv[3] != T(0)) return;
+    const T rho = pv[0];
+    if (krylov_stop<T>(it == 0, it, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
     T beta = 0;
     if (it > 0) {
         T rho_old = ks[KS_RHO + ((it + 1) & 1)];
         beta = rho_old != T(0) ? rho / rho_old : T(0);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
-    const int i = krow<L>();
-    const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
-    T s;
-    if (it == 0)
-        s = row_sum<T, L>(rb, re, ci, val, [&](int j) { return z[j]; });
-    else
-        s = row_sum<T, L>(rb, re, ci, val, [&](int j) { return z[j] + beta * pold[j]; });
     T acc[1] = {0};
     if (in && klane<L>() == 0) {
-        T pi = it == 0 ? z[i] : z[i] + beta * pold[i];
+        const T pi = it == 0 ? zi : zi + beta * poi;
+        const T qi = it == 0 ? sz[0] : sz[0] + beta * sz[1];
         pnew[i] = pi;
-        q[i] = s;
-        acc[0] = pi * s;
+        q[i] = qi;
+        acc[0] = pi * qi;
     }
     const int so[1] = {KP_PQ};
     write_partials<T, 1>(acc, part, nb, so);
@@ -221,19 +278,29 @@ __global__ void __launch_bounds__(KNT) k_kcg_update(int n, int it, T *__restrict
                                                     const T *__restrict__ p, const T *__restrict__ q,
                                                     const T *__restrict__ dinv, T *__restrict__ z,
                                                     T *__restrict__ part, int nb, const T *ks, const int *ki) {
-    if (krylov_done(ki)) return;
-    T s1[1];
     const int sl[1] = {KP_PQ};
-    sum_partials<T, 1>(part, nb, sl, s1);
-    const T rho = ks[KS_RHO + (it & 1)];
-    const T alpha = s1[0] != T(0) ? rho / s1[0] : T(0);
+    __shared__ T pv[2];
+    reduce_partials<T, 1>(part, nb, sl, ki, pv);
     const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
+    const bool own = (int)threadIdx.x < KNT / L && i < n;
+    T xi = 0, ri = 0, pi = 0, qi = 0, di = 0;
+    if (own) {
+        xi = x[i];
+        ri = r[i];
+        pi = p[i];
+        qi = q[i];
+        di = dinv[i];
+    }
+    kblock_sync();
+    if (pv[1] != T(0)) return;
+    const T rho = ks[KS_RHO + (it & 1)];
+    const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
     T acc[2] = {0, 0};
-    if ((int)threadIdx.x < KNT / L && i < n) {
-        x[i] = x[i] + alpha * p[i];
-        T ri = r[i] - alpha * q[i];
+    if (own) {
+        x[i] = xi + alpha * pi;
+        ri = ri - alpha * qi;
         r[i] = ri;
-        T zi = dinv[i] * ri;
+        T zi = di * ri;
         z[i] = zi;
         acc[0] = ri * zi;
         acc[1] = ri * ri;
@@ -243,108 +310,135 @@ __global__ void __launch_bounds__(KNT) k_kcg_update(int n, int it, T *__restrict
 }
 
 // -------------------------------------------------------------- BiCGStab
-// launch 1: stopping test; rho = rh.r; beta = (rho/rho_old)(alpha/omega);
-// p = r + beta y (y = p_old - omega v_old); v = (A D^-1) p; partial rh.v
+// launch A of iteration `it`.  it = 0: stopping test on the initial
+// residual, rho = rh.r, p = r, v = (A D^-1) r.  it > 0: omega = t.s / t.t of
+// iteration it-1, x += alpha dinv p_old + omega dinv s, r = s - omega t,
+// rho = rh.s - omega rh.t, beta = (rho / rho_old)(alpha / omega),
+// p = r + beta (p_old - omega v_old), v = (A D^-1) p.  Partials rh.v, r.r.
 template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kbcg_v(int n, int it, int maxit, T rtol, T atol,
+__global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol, T atol,
                                                 const int *__restrict__ rp, const int *__restrict__ ci,
-                                                const T *__restrict__ vhat, const T *__restrict__ r,
-                                                const T *__restrict__ rh, const T *__restrict__ y,
-                                                T *__restrict__ p, T *__restrict__ v, T *__restrict__ part, int nb,
-                                                T *ks, int *ki) {
-    if (krylov_done(ki)) return;
-    T s3[3];
-    const int sl[3] = {KP_RZ, KP_RR, KP_BB};
-    sum_partials<T, 3>(part, nb, sl, s3);
-    const T rho = s3[0];
-    T beta = 0;
-    bool breakdown = false;
-    if (it > 0) {
-        const int o = (it + 1) & 1;
-        const T rho_old = ks[KS_RHO + o], om = ks[KS_OMEGA + o];
-        breakdown = rho == T(0) || rho_old == T(0) || om == T(0);
-        if (!breakdown) beta = (rho / rho_old) * (ks[KS_ALPHA + o] / om);
-    }
-    if (krylov_stop<T>(it, maxit, s3[1], s3[2], rtol, atol, breakdown, ks, ki)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
-    const int i = krow<L>();
-    const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
-    T s;
-    if (it == 0)
-        s = row_sum<T, L>(rb, re, ci, vhat, [&](int j) { return r[j]; });
-    else
-        s = row_sum<T, L>(rb, re, ci, vhat, [&](int j) { return r[j] + beta * y[j]; });
-    T acc[1] = {0};
-    if (in && klane<L>() == 0) {
-        p[i] = it == 0 ? r[i] : r[i] + beta * y[i];
-        v[i] = s;
-        acc[0] = rh[i] * s;
-    }
-    const int so[1] = {KP_RV};
-    write_partials<T, 1>(acc, part, nb, so);
-}
-
-// launch 2: alpha = rho / rh.v; s = r - alpha v; t = (A D^-1) s;
-// partials t.s, t.t
-template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kbcg_t(int n, int it, const int *__restrict__ rp,
-                                                const int *__restrict__ ci, const T *__restrict__ vhat,
-                                                const T *__restrict__ r, const T *__restrict__ v, T *__restrict__ s,
-                                                T *__restrict__ t, T *__restrict__ part, int nb, T *ks,
-                                                const int *ki) {
-    if (krylov_done(ki)) return;
-    T s1[1];
-    const int sl[1] = {KP_RV};
-    sum_partials<T, 1>(part, nb, sl, s1);
-    const T rho = ks[KS_RHO + (it & 1)];
-    const T alpha = s1[0] != T(0) ? rho / s1[0] : T(0);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
-    const int i = krow<L>();
-    const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
-    T a = row_sum<T, L>(rb, re, ci, vhat, [&](int j) { return r[j] - alpha * v[j]; });
-    T acc[2] = {0, 0};
-    if (in && klane<L>() == 0) {
-        T si = r[i] - alpha * v[i];
-        s[i] = si;
-        t[i] = a;
-        acc[0] = a * si;
-        acc[1] = a * a;
-    }
-    const int so[2] = {KP_TS, KP_TT};
-    write_partials<T, 2>(acc, part, nb, so);
-}
-
-// launch 3: omega = t.s / t.t; x += alpha dinv p + omega dinv s;
-// r = s - omega t; y = p - omega v; partials rh.r, r.r
-template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kbcg_x(int n, int it, T *__restrict__ x, T *__restrict__ r,
-                                                const T *__restrict__ rh, const T *__restrict__ dinv,
-                                                const T *__restrict__ p, const T *__restrict__ v,
+                                                const T *__restrict__ vhat, const T *__restrict__ dinv,
+                                                T *__restrict__ x, T *__restrict__ r, const T *__restrict__ rh,
                                                 const T *__restrict__ s, const T *__restrict__ t,
-                                                T *__restrict__ y, T *__restrict__ part, int nb, T *ks,
-                                                const int *ki) {
-    if (krylov_done(ki)) return;
-    T s2[2];
-    const int sl[2] = {KP_TS, KP_TT};
-    sum_partials<T, 2>(part, nb, sl, s2);
-    const T omega = s2[1] > T(0) ? s2[0] / s2[1] : T(0);
-    const T alpha = ks[KS_ALPHA + (it & 1)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_OMEGA + (it & 1)] = omega;
-    const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
+                                                const T *__restrict__ pold, const T *__restrict__ vold,
+                                                T *__restrict__ pnew, T *__restrict__ vnew, T *__restrict__ part,
+                                                int nb, T *ks, int *ki) {
+    const int i = krow<L>();
+    const bool in = i < n;
+    const bool own = in && klane<L>() == 0;
+    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
     T acc[2] = {0, 0};
-    if ((int)threadIdx.x < KNT / L && i < n) {
-        const T di = dinv[i], pi = p[i], si = s[i];
-        x[i] = x[i] + alpha * (di * pi) + omega * (di * si);
-        T ri = si - omega * t[i];
-        r[i] = ri;
-        y[i] = pi - omega * v[i];
-        acc[0] = rh[i] * ri;
-        acc[1] = ri * ri;
+    if (it == 0) {
+        const int sl[3] = {KP_RZ, KP_RR, KP_BB};
+        __shared__ T pv[4];
+        reduce_partials<T, 3>(part, nb, sl, ki, pv);
+        const T *const vr[1] = {r};
+        T sr[1];
+        row_sums<T, L, 1>(rb, re, ci, vhat, vr, sr);
+        const T ri = in ? r[i] : T(0), rhi = in ? rh[i] : T(0);
+        kblock_sync();
+        if (pv[3] != T(0)) return;
+        if (krylov_stop<T>(true, 0, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO] = pv[0];
+        if (own) {
+            pnew[i] = ri;
+            vnew[i] = sr[0];
+            acc[0] = rhi * sr[0];
+            acc[1] = ri * ri;
+        }
+    } else {
+        const int sl[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
+        __shared__ T pv[5];
+        reduce_partials<T, 4>(part, nb, sl, ki, pv, KI_BRK + ((it + 1) & 1));
+        const T *const vv[4] = {s, t, pold, vold};
+        T sv[4];
+        row_sums<T, L, 4>(rb, re, ci, vhat, vv, sv);
+        T si = 0, ti = 0, poi = 0, voi = 0, xi = 0, di = 0, rhi = 0;
+        if (own) {
+            si = s[i];
+            ti = t[i];
+            poi = pold[i];
+            voi = vold[i];
+            xi = x[i];
+            di = dinv[i];
+            rhi = rh[i];
+        }
+        kblock_sync();
+        if (pv[4] != T(0)) return;
+        const int o = (it + 1) & 1;  // scalars of iteration it-1
+        const T omega = pv[1] > T(0) ? pv[0] / pv[1] : T(0);
+        const T alpha = ks[KS_ALPHA + o], rho_old = ks[KS_RHO + o];
+        const T rho = pv[2] - omega * pv[3];
+        const bool breakdown = rho == T(0) || rho_old == T(0) || omega == T(0);
+        const T beta = breakdown ? T(0) : (rho / rho_old) * (alpha / omega);
+        T ri = 0;
+        if (own) {
+            x[i] = xi + alpha * (di * poi) + omega * (di * si);
+            ri = si - omega * ti;
+            r[i] = ri;
+        }
+        if (breakdown) {
+            // x holds this iteration's update; the host checks the true residual
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ki[KI_ITERS] = it;
+                ki[KI_CONV] = 0;
+                ki[KI_BRK + (it & 1)] = 1;
+            }
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
+        if (own) {
+            const T vi = (sv[0] - omega * sv[1]) + beta * (sv[2] - omega * sv[3]);
+            pnew[i] = ri + beta * (poi - omega * voi);
+            vnew[i] = vi;
+            acc[0] = rhi * vi;
+            acc[1] = ri * ri;
+        }
     }
-    const int so[2] = {KP_RZ, KP_RR};
+    // iteration 0 reads the r.r partials this launch would overwrite (with
+    // the same values): it writes its r.r to an unused slot instead
+    const int so[2] = {KP_RV, it == 0 ? KP_PQ : KP_RR};
     write_partials<T, 2>(acc, part, nb, so);
+}
+
+// launch B: stopping test on r.r; alpha = rho / rh.v; s = r - alpha v;
+// t = (A D^-1) s; partials t.s, t.t, rh.s, rh.t
+template <typename T, int L>
+__global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, const int *__restrict__ rp,
+                                                const int *__restrict__ ci, const T *__restrict__ vhat,
+                                                const T *__restrict__ r, const T *__restrict__ rh,
+                                                const T *__restrict__ v, T *__restrict__ s, T *__restrict__ t,
+                                                T *__restrict__ part, int nb, T *ks, int *ki) {
+    const int sl[2] = {KP_RV, KP_RR};
+    __shared__ T pv[3];
+    reduce_partials<T, 2>(part, nb, sl, ki, pv, KI_BRK + (it & 1));
+    const int i = krow<L>();
+    const bool in = i < n;
+    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
+    const T *const vv[2] = {r, v};
+    T sv[2];
+    row_sums<T, L, 2>(rb, re, ci, vhat, vv, sv);
+    const T ri = in ? r[i] : T(0), vi = in ? v[i] : T(0), rhi = in ? rh[i] : T(0);
+    kblock_sync();
+    if (pv[2] != T(0)) return;
+    if (it > 0 && krylov_stop<T>(false, it, maxit, pv[1], T(0), T(0), T(0), false, ks, ki)) return;
+    const T rho = ks[KS_RHO + (it & 1)];
+    const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
+    T acc[4] = {0, 0, 0, 0};
+    if (in && klane<L>() == 0) {
+        const T si = ri - alpha * vi;
+        const T ti = sv[0] - alpha * sv[1];
+        s[i] = si;
+        t[i] = ti;
+        acc[0] = ti * si;
+        acc[1] = ti * ti;
+        acc[2] = rhi * si;
+        acc[3] = rhi * ti;
+    }
+    const int so[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
+    write_partials<T, 4>(acc, part, nb, so);
 }
 
 // ---------------------------------------------------------------- driver
@@ -372,8 +466,8 @@ class FusedKrylov {
   public:
     int n = 0, nb = 0, lanes = 1;
     const int *rp = nullptr, *ci = nullptr;
-    KBuf<T> val, vhat, dinv;                        // A, A D^-1, D^-1
-    KBuf<T> r, rh, z, p, p2, q, v, y, s, t, part, ks;
+    KBuf<T> val, vhat, dinv;  // A, A D^-1, D^-1
+    KBuf<T> r, rh, z, p, p2, q, v, v2, s, t, part, ks;
     KBuf<int> ki;
     int pred = 0;  // iterations of the last solve (first chunk)
 
@@ -396,13 +490,10 @@ class FusedKrylov {
             th[k] = (T)(hv[k] * hd[hci[k]]);
         }
         for (int i = 0; i < n; i++) td[i] = (T)hd[i];
-        val.alloc(tv.size());
-        vhat.alloc(th.size());
-        dinv.alloc(n);
         up(val, tv, st);
         up(vhat, th, st);
         up(dinv, td, st);
-        for (KBuf<T> *b : {&r, &rh, &z, &p, &p2, &q, &v, &y, &s, &t}) b->alloc(n);
+        for (KBuf<T> *b : {&r, &rh, &z, &p, &p2, &q, &v, &v2, &s, &t}) b->alloc(n);
         ks.alloc(KS_N);
         ki.alloc(KI_N);
         pred = 0;
@@ -429,8 +520,31 @@ class FusedKrylov {
         }
     }
 
+    // Times `reps` launches of BiCGStab launch B, t = (A D^-1)(r - alpha v)
+    // (the solve's SpMV launch shape), on the current vectors with the flags
+    // cleared; call between solves only.  Returns the average ms and the
+    // algorithmic bytes of one launch: CSR structure and values, r, v and rh
+    // read once, s and t written once, the partials.
+    float probe(int reps, long *bytes, hipStream_t st) {
+        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+        float ms = 0;
+        switch (lanes) {
+        case 1: ms = time_b<1>(reps, st); break;
+        case 2: ms = time_b<2>(reps, st); break;
+        case 4: ms = time_b<4>(reps, st); break;
+        default: ms = time_b<8>(reps, st); break;
+        }
+        long nnz = 0;
+        int last = 0;
+        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
+        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 5 * sizeof(T) +
+                 (long)nb * 6 * sizeof(T);
+        return ms;
+    }
+
   private:
     static void up(KBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
+        d.alloc(h.size());
         if (!h.empty() &&
             hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
             throw std::runtime_error("hipMemcpyAsync failed");
@@ -439,7 +553,6 @@ class FusedKrylov {
     int blocks() const {
         return std::max(1, (n + KNT / L - 1) / (KNT / L));
     }
-    void size_partials() { part.alloc((size_t)KP_N * nb); }
 
     template <typename TB, int L>
     void launch_init(bool cg, const TB *b, T *x, hipStream_t st) {
@@ -449,55 +562,57 @@ class FusedKrylov {
     template <int L>
     void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
         for (int it = it0; it < it0 + count; it++) {
+            T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
             if (cg) {
-                T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
                 k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, z.p, pold, pnew, q.p,
                                                      part.p, nb, ks.p, ki.p);
                 k_kcg_update<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb, ks.p, ki.p);
             } else {
-                k_kbcg_v<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, vhat.p, r.p, rh.p, y.p, p.p, v.p,
-                                                   part.p, nb, ks.p, ki.p);
-                k_kbcg_t<T, L><<<nb, KNT, 0, st>>>(n, it, rp, ci, vhat.p, r.p, v.p, s.p, t.p, part.p, nb, ks.p, ki.p);
-                k_kbcg_x<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, rh.p, dinv.p, p.p, v.p, s.p, t.p, y.p, part.p, nb,
-                                                   ks.p, ki.p);
+                T *vold = (it & 1) ? v.p : v2.p, *vnew = (it & 1) ? v2.p : v.p;
+                k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, vhat.p, dinv.p, x, r.p, rh.p, s.p,
+                                                   t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p);
+                k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rp, ci, vhat.p, r.p, rh.p, vnew, s.p, t.p, part.p,
+                                                   nb, ks.p, ki.p);
             }
         }
     }
     template <int L>
-    float time_lanes(hipStream_t st) {
-        lanes = L;
-        nb = blocks<L>();
-        size_partials();
-        // t = (A D^-1)(r - alpha v) on zero vectors, flags cleared: reads and
-        // writes only buffers of this solver
-        k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, r.p, T(0));
-        k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, v.p, T(0));
-        k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
-        k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
-        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+    float time_b(int reps, hipStream_t st) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
-        for (int rep = 0; rep < 2; rep++)
-            k_kbcg_t<T, L><<<nb, KNT, 0, st>>>(n, 0, rp, ci, vhat.p, r.p, v.p, s.p, t.p, part.p, nb, ks.p, ki.p);
         (void)hipEventRecord(a, st);
-        for (int rep = 0; rep < 20; rep++)
-            k_kbcg_t<T, L><<<nb, KNT, 0, st>>>(n, 0, rp, ci, vhat.p, r.p, v.p, s.p, t.p, part.p, nb, ks.p, ki.p);
+        for (int rep = 0; rep < reps; rep++)
+            k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, 0, 1 << 30, rp, ci, vhat.p, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
+                                               ks.p, ki.p);
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
         float ms = 0;
         (void)hipEventElapsedTime(&ms, a, b);
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        return ms;
+        return reps > 0 ? ms / reps : 0.0f;
+    }
+    template <int L>
+    float time_lanes(hipStream_t st) {
+        lanes = L;
+        nb = blocks<L>();
+        part.alloc((size_t)KP_N * nb);
+        // launch B on zero vectors with the flags cleared: reads and writes
+        // only buffers of this solver
+        for (KBuf<T> *b : {&r, &rh, &v}) k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, b->p, T(0));
+        k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
+        k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
+        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+        time_b<L>(2, st);
+        return time_b<L>(20, st);
     }
     // picks the lanes per row with the fastest SpMV launch
     void tune(hipStream_t st) {
+        const float tl[4] = {time_lanes<1>(st), time_lanes<2>(st), time_lanes<4>(st), time_lanes<8>(st)};
+        const int ll[4] = {1, 2, 4, 8};
         int best = 1;
         float tb = 1e30f;
-        const float t1 = time_lanes<1>(st), t2 = time_lanes<2>(st), t4 = time_lanes<4>(st), t8 = time_lanes<8>(st);
-        const float tl[4] = {t1, t2, t4, t8};
-        const int ll[4] = {1, 2, 4, 8};
         for (int k = 0; k < 4; k++)
             if (tl[k] < tb) {
                 tb = tl[k];
@@ -509,7 +624,7 @@ class FusedKrylov {
         }
         lanes = best;
         nb = best == 1 ? blocks<1>() : best == 2 ? blocks<2>() : best == 4 ? blocks<4>() : blocks<8>();
-        size_partials();
+        part.alloc((size_t)KP_N * nb);
     }
 };
 

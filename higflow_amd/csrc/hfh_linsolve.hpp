@@ -356,6 +356,8 @@ struct SolveStats {
     double resid = 0.0;  // ||b - Ax|| / ||b||
     int converged = 0;
     int outer = 0;       // refinement steps (mixed)
+    int fallback = 0;    // mixed: the solve finished in FP64
+    int restarts = 0;    // fused: restarts on the true residual
 };
 
 // A sparse system on one device.
@@ -375,6 +377,9 @@ class DeviceSystem {
     FusedKrylov<double> k64;
     FusedKrylov<float> k32;
     bool fused = true;
+    int fp64_solves_left = 0;  // mixed: solves to run in FP64 after an FP32 failure
+    int max_restarts = 1;      // fused64: restarts on the true residual
+    DevBuf<double> xbak;
 
     DeviceSystem() = default;
     DeviceSystem(const DeviceSystem &) = delete;
@@ -411,6 +416,8 @@ class DeviceSystem {
         if (!sc.p) sc.alloc(S_NSC);
         if (!fsc.p) fsc.alloc(S_NSC);
         if (!hpin_) HFH_CHECK(hipHostMalloc((void **)&hpin_, 512, hipHostMallocDefault));
+        if ((int)xbak.n < n) xbak.alloc(n);
+        if (const char *er = std::getenv("HFH_RESTARTS")) max_restarts = std::max(0, std::atoi(er));
         k64.set(n, rp.p, ci.p, hrp, hci, hv, st);
         k32.set(n, rp.p, ci.p, hrp, hci, hv, st);
         // HFH_KRYLOV=classic selects the one-kernel-per-operation iterations
@@ -427,6 +434,7 @@ class DeviceSystem {
         HFH_CHECK(hipMemcpyAsync(hpin_ + 64, k.ks.p, KS_N * sizeof(T), hipMemcpyDeviceToHost, st));
         HFH_CHECK(hipStreamSynchronize(st));
         std::memcpy(hi, hpin_, KI_N * sizeof(int));
+        if (hi[KI_BRK] || hi[KI_BRK + 1]) hi[KI_DONE] = 1;  // breakdown: stopped, not converged
         const T *t = reinterpret_cast<const T *>(hpin_ + 64);
         for (int q = 0; q < KS_N; q++) hs[q] = (double)t[q];
     }
@@ -449,15 +457,39 @@ class DeviceSystem {
         return used;
     }
 
-    // FP64 fused solve: Jacobi-PCG (spd) or BiCGStab, warm-started from x
+    // FP64 fused solve: Jacobi-PCG (spd) or BiCGStab, warm-started from x.
+    // The stopping test runs on the recursively updated residual; once it
+    // passes, the true residual b - A x is checked and the iteration
+    // restarted from x (at most twice) while it misses the tolerance.
     SolveStats fused64(bool spd, const double *b, double *x, double rtol, double atol, int maxit, hipStream_t st) {
         int hi[KI_N];
-        double hs[KS_N];
-        k64.init<double>(spd, b, x, st);
+        double hs[KS_N], h[S_NSC];
         SolveStats S;
-        S.iters = run_fused<double>(k64, spd, maxit, rtol, atol, x, hi, hs, st);
-        S.converged = hi[KI_DONE] && hi[KI_CONV];
-        S.resid = hs[KS_BN] > 0 ? std::sqrt(hs[KS_RR]) / hs[KS_BN] : std::sqrt(hs[KS_RR]);
+        double best = -1.0;  // true residual norm of the kept x
+        for (int restart = 0; restart <= max_restarts; restart++) {
+            if (restart > 0)  // keep the current x in case the restart does worse
+                HFH_CHECK(hipMemcpyAsync(xbak.p, x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            k64.init<double>(spd, b, x, st);
+            S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
+            k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
+            HFH_CHECK(hipMemcpyAsync(hpin_ + 256, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+            std::memcpy(h, hpin_ + 256, sizeof h);
+            const double bn = std::sqrt(h[S_BB]), rn = std::sqrt(h[S_RR]);
+            const double tol = std::max(rtol * bn, atol);
+            if (restart > 0 && !(rn < best)) {  // no better: restore
+                HFH_CHECK(hipMemcpyAsync(x, xbak.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                break;
+            }
+            best = rn;
+            S.resid = bn > 0 ? rn / bn : rn;
+            S.converged = rn <= tol;
+            // restart only when the recursion claims convergence the true
+            // residual misses by more than a factor 10
+            if (S.converged || !(hi[KI_DONE] && hi[KI_CONV]) || rn <= 10.0 * tol || S.iters >= maxit) break;
+            S.restarts++;
+        }
         return S;
     }
 
@@ -467,6 +499,13 @@ class DeviceSystem {
     // error).
     SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                            int maxit, hipStream_t st) {
+        if (fp64_solves_left > 0) {
+            // the FP32 corrections failed on this system recently
+            fp64_solves_left--;
+            SolveStats S = fused64(spd, b, x, rtol, atol, maxit, st);
+            S.fallback = 1;
+            return S;
+        }
         SolveStats S;
         double h[S_NSC];
         int hi[KI_N];
@@ -486,9 +525,21 @@ class DeviceSystem {
             k_convert<float, double><<<nb, BLOCK, 0, st>>>(n, r.p, b32.p);
             k_fill<float><<<nb, BLOCK, 0, st>>>(n, x32.p, 0.0f);
             k32.init<float>(spd, b32.p, x32.p, st);
-            const int used = run_fused<float>(k32, spd, maxit - inner_total, (float)inner_rtol, 0.0f, x32.p, hi, hs,
-                                              st);
+            // an inner solve may take at most twice the FP64 solve's count
+            const int cap = std::min(maxit - inner_total, std::max(200, 2 * k64.pred + 50));
+            const int used = run_fused<float>(k32, spd, cap, (float)inner_rtol, 0.0f, x32.p, hi, hs, st);
             inner_total += std::max(used, 1);
+            if (!(hi[KI_DONE] && hi[KI_CONV])) {
+                // FP32 Krylov cannot reach inner_rtol on this system (too
+                // ill-conditioned for single precision): finish in FP64 from
+                // the current x and keep FP64 for the next solves
+                fp64_solves_left = 20;
+                SolveStats F = fused64(spd, b, x, rtol, atol, maxit, st);
+                F.iters += inner_total;
+                F.outer = outer;
+                F.fallback = 1;
+                return F;
+            }
             k_add_conv<double, float><<<nb, BLOCK, 0, st>>>(n, x, x32.p);
             outer++;
         }

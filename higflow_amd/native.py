@@ -96,6 +96,7 @@ def load():
     L.hfh_stage_bytes.restype = C.c_long
     L.hfh_sync.argtypes = [vp]
     L.hfh_operator_csr.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip, dp]
+    L.hfh_solve_stats.argtypes = [vp, C.c_int, ip, dp]
     L.hfh_set_timing.argtypes = [vp, C.c_int]
     L.hfh_probe.argtypes = [vp, C.c_int, C.c_int, dp, lp]
     L.hfh_version.argtypes = []
@@ -217,6 +218,15 @@ class Flow:
         r = C.c_double(0)
         conv = self.L.hfh_last_solve(self.h, C.byref(it), C.byref(r), C.byref(outer))
         return conv, it.value, r.value, outer.value
+
+    def solve_stats(self, which=0):
+        """Last solve of a system (0 pressure, 1+d velocity component d):
+        dict of converged, iters, outer, fallback, restarts, resid."""
+        info = (C.c_int * 4)()
+        r = C.c_double(0)
+        conv = _check(self.L, self.L.hfh_solve_stats(self.h, which, info, C.byref(r)), "hfh_solve_stats")
+        return dict(converged=conv, iters=info[0], outer=info[1], fallback=info[2], restarts=info[3],
+                    resid=r.value)
 
     def stencil(self, domain, dim, center, x, alpha=1.0):
         ids = (C.c_int * 4096)()

@@ -136,6 +136,12 @@ int hfh_set_time(hfh_solver *s, double t);
 double hfh_get_time(hfh_solver *s);
 int hfh_pinned_cell(hfh_solver *s);
 int hfh_last_solve(hfh_solver *s, int *iters, double *resid, int *outer);
+/* the last solve of a system (0 pressure, 1+d semi-implicit velocity
+ * component d), as the solver's SolverIterations / SolverResidual /
+ * converged (higtree/src/solver.h:80-82): info = {iterations, refinement
+ * steps, finished-in-FP64 flag (mixed), true-residual restarts}; returns
+ * 1 converged, 0 not converged, <0 error */
+int hfh_solve_stats(hfh_solver *s, int which, int *info, double *resid);
 
 /* sd_get_stencil / sfd_get_stencil on the library's host mesh: domain 0
  * pressure cells, 1 velocity facets of component dim, 2 tensor cells.
@@ -153,8 +159,9 @@ long hfh_stage_bytes(hfh_solver *s, int stage);
 int hfh_set_timing(hfh_solver *s, int on);
 /* launches one step kernel `reps` times between hipEvents on the library
  * stream: 0 pressure SpMV (+dot), 1 momentum, 2 momentum interpolation rows,
- * 3 divergence rows, 4 projection rows, 5 constitutive.  Average ms per
- * launch and the algorithmic bytes of one launch. */
+ * 3 divergence rows, 4 projection rows, 5 constitutive, 6 / 7 the fused
+ * pressure BiCGStab SpMV launch in FP64 / FP32, 8 a plain copy of 64 MiB.
+ * Average ms per launch and the algorithmic bytes of one launch. */
 int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
 int hfh_sync(hfh_solver *s);
 /* the assembled systems as CSR: which 0 pressure (the matrix slv_set_Ai

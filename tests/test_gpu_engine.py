@@ -66,3 +66,44 @@ def test_impose_value_and_first_gid():
     b2 = b.copy()
     b2[10] = 3.25
     assert np.allclose(s.get_x(), np.linalg.solve(A2, b2), atol=1e-9)
+
+
+def _contraction_pressure(levels, h0, length):
+    # the assembled pressure system of a 4:1 contraction (host-only build)
+    import os
+    import scipy.sparse as sp
+    from higflow_amd import cases
+    os.environ["HFH_HOST_ONLY"] = "1"
+    try:
+        f = native.Flow(cases.contraction(levels=levels, h0=h0, length=length, radius=1.0))
+        rp, ci, v, sym = f.operator(0)
+        f.close()
+    finally:
+        del os.environ["HFH_HOST_ONLY"]
+    return sp.csr_matrix((v, ci, rp))
+
+
+@pytest.mark.parametrize("coarse", ["0", "1"])
+def test_fused_bicgstab_on_contraction_pressure(coarse, monkeypatch):
+    # fused BiCGStab (with and without the workgroup-block coarse
+    # correction) on a real nonsymmetric AMR pressure operator, against a
+    # direct solve
+    import scipy.sparse.linalg as sla
+    A = _contraction_pressure(2, 0.25, 8.0)
+    n = A.shape[0]
+    x0 = np.random.default_rng(5).normal(size=n)
+    b = A @ x0
+    monkeypatch.setenv("HFH_COARSE", coarse)
+    s = native.LinearSolver(0, n)
+    s.set_tolerances(1e-12, 0.0, 20000)
+    for i in range(n):
+        s.set_Ai(i, A.indices[A.indptr[i]:A.indptr[i + 1]], A.data[A.indptr[i]:A.indptr[i + 1]])
+        s.set_bi(i, b[i])
+    s.assemble()
+    s.solve()
+    x = s.get_x()
+    conv, iters, resid = s.stats()[:3]
+    print("coarse", coarse, "n", n, "iters", iters, "resid", resid)
+    xd = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-11
+    assert np.abs(x - xd).max() / np.abs(xd).max() < 1e-7
