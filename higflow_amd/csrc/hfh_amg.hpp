@@ -1,0 +1,608 @@
+// Smoothed-aggregation algebraic multigrid for the pressure systems:
+// host set-up (aggregation, prolongator smoothing, Galerkin products, dense
+// coarsest inverse) and the device V-cycle used as the right preconditioner
+// of BiCGStab.
+//
+// Set-up (once per matrix):
+//   strength   |a_ij| >= theta sqrt(|a_ii a_jj|) on (A + A^T)/2, theta 0.08
+//   aggregates greedy: a node whose strong neighbours are all free seeds an
+//              aggregate with them; left-over nodes join a neighbouring
+//              aggregate (or form their own)
+//   P          (I - 4/(3 lambda) D^-1 A) P_tent, P_tent piecewise constant,
+//              lambda the spectral radius of D^-1 A by 15 power steps
+//   R = P^T, A_c = R A P, until at most `coarse_max` unknowns remain; the
+//   coarsest level is inverted densely (Gauss-Jordan, partial pivoting).
+// V(1,1) cycle with damped Jacobi (omega 0.6):
+//   x = omega D^-1 b; r = b - A x; b_c = R r; x_c = V(b_c); x += P x_c;
+//   x += omega D^-1 (b - A x)
+// Each kernel launch handles one such step for one level; rows run on L
+// lanes as in the Krylov kernels.  Every launch first reads the solve's done
+// flag and returns when a previous launch has set it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "hfh_krylov.hpp"
+
+namespace hfh {
+
+// ------------------------------------------------------------- host CSR
+struct HCsr {
+    int nrow = 0, ncol = 0;
+    std::vector<int> rp{0}, ci;
+    std::vector<double> v;
+};
+
+// C = A B (row-wise with a dense accumulator over B's columns)
+inline HCsr csr_mul(const HCsr &A, const HCsr &B) {
+    HCsr C;
+    C.nrow = A.nrow;
+    C.ncol = B.ncol;
+    C.rp.assign(1, 0);
+    std::vector<double> acc(B.ncol, 0.0);
+    std::vector<int> mark(B.ncol, -1), cols;
+    for (int i = 0; i < A.nrow; i++) {
+        cols.clear();
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int j = A.ci[k];
+            const double a = A.v[k];
+            for (int q = B.rp[j]; q < B.rp[j + 1]; q++) {
+                const int c = B.ci[q];
+                if (mark[c] != i) {
+                    mark[c] = i;
+                    acc[c] = 0.0;
+                    cols.push_back(c);
+                }
+                acc[c] += a * B.v[q];
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int c : cols) {
+            C.ci.push_back(c);
+            C.v.push_back(acc[c]);
+        }
+        C.rp.push_back((int)C.ci.size());
+    }
+    return C;
+}
+
+inline HCsr csr_transpose(const HCsr &A) {
+    HCsr T;
+    T.nrow = A.ncol;
+    T.ncol = A.nrow;
+    T.rp.assign(A.ncol + 1, 0);
+    for (int c : A.ci) T.rp[c + 1]++;
+    for (int i = 0; i < A.ncol; i++) T.rp[i + 1] += T.rp[i];
+    T.ci.resize(A.ci.size());
+    T.v.resize(A.v.size());
+    std::vector<int> pos(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.nrow; i++)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int p = pos[A.ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[k];
+        }
+    return T;
+}
+
+inline std::vector<double> csr_diag_inv(const HCsr &A) {
+    std::vector<double> d(A.nrow, 1.0);
+    for (int i = 0; i < A.nrow; i++)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++)
+            if (A.ci[k] == i && A.v[k] != 0.0) d[i] = 1.0 / A.v[k];
+    return d;
+}
+
+// greedy aggregation on the symmetrised strength graph; returns the
+// aggregate of each row and the aggregate count
+inline int sa_aggregate(const HCsr &A, double theta, std::vector<int> &agg) {
+    const int n = A.nrow;
+    std::vector<double> dg(n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++)
+            if (A.ci[k] == i) dg[i] = std::sqrt(std::fabs(A.v[k]));
+    // strong neighbours: |(a_ij + a_ji)/2| >= theta sqrt(|a_ii a_jj|)
+    HCsr At = csr_transpose(A);
+    std::vector<std::vector<int>> S(n);
+    std::vector<double> acc(n, 0.0);
+    std::vector<int> mark(n, -1), cols;
+    for (int i = 0; i < n; i++) {
+        cols.clear();
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int j = A.ci[k];
+            if (mark[j] != i) {
+                mark[j] = i;
+                acc[j] = 0.0;
+                cols.push_back(j);
+            }
+            acc[j] += 0.5 * A.v[k];
+        }
+        for (int k = At.rp[i]; k < At.rp[i + 1]; k++) {
+            const int j = At.ci[k];
+            if (mark[j] != i) {
+                mark[j] = i;
+                acc[j] = 0.0;
+                cols.push_back(j);
+            }
+            acc[j] += 0.5 * At.v[k];
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols)
+            if (j != i && std::fabs(acc[j]) >= theta * dg[i] * dg[j]) S[i].push_back(j);
+    }
+    agg.assign(n, -1);
+    int na = 0;
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        bool free = true;
+        for (int j : S[i])
+            if (agg[j] >= 0) {
+                free = false;
+                break;
+            }
+        if (!free) continue;
+        agg[i] = na;
+        for (int j : S[i]) agg[j] = na;
+        na++;
+    }
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        int a = -1;
+        for (int j : S[i])
+            if (agg[j] >= 0) {
+                a = agg[j];
+                break;
+            }
+        agg[i] = a >= 0 ? a : na++;
+    }
+    return na;
+}
+
+struct AmgHostLevel {
+    HCsr A, P, R;
+    std::vector<double> dinv;
+};
+
+// Builds the hierarchy; returns false (no hierarchy) when the coarsening
+// stalls or the coarsest matrix cannot be inverted.
+inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<double> &cinv, int &nc,
+                     int coarse_max = 400, double theta = 0.08, int max_levels = 10) {
+    lv.clear();
+    HCsr A = A0;
+    while (A.nrow > coarse_max && (int)lv.size() < max_levels) {
+        AmgHostLevel L;
+        L.A = A;
+        L.dinv = csr_diag_inv(A);
+        std::vector<int> agg;
+        const int na = sa_aggregate(A, theta, agg);
+        if (na >= A.nrow || na == 0) return false;  // no coarsening
+        // spectral radius of D^-1 A (power iteration from a fixed start)
+        std::vector<double> x(A.nrow), y(A.nrow);
+        for (int i = 0; i < A.nrow; i++) x[i] = 1.0 + 0.5 * std::sin(0.7 * i);
+        double lam = 1.0;
+        for (int it = 0; it < 15; it++) {
+            double nn = 0;
+            for (int i = 0; i < A.nrow; i++) {
+                double s = 0;
+                for (int k = A.rp[i]; k < A.rp[i + 1]; k++) s += A.v[k] * x[A.ci[k]];
+                y[i] = L.dinv[i] * s;
+                nn += y[i] * y[i];
+            }
+            lam = std::sqrt(nn);
+            if (!(lam > 0)) return false;
+            for (int i = 0; i < A.nrow; i++) x[i] = y[i] / lam;
+        }
+        // P = (I - w D^-1 A) P_tent, w = 4 / (3 lambda)
+        const double w = 4.0 / (3.0 * lam);
+        HCsr Pt;
+        Pt.nrow = A.nrow;
+        Pt.ncol = na;
+        Pt.rp.resize(A.nrow + 1);
+        Pt.ci.resize(A.nrow);
+        Pt.v.assign(A.nrow, 1.0);
+        for (int i = 0; i < A.nrow; i++) {
+            Pt.rp[i + 1] = i + 1;
+            Pt.ci[i] = agg[i];
+        }
+        HCsr S = A;  // S = I - w D^-1 A
+        for (int i = 0; i < A.nrow; i++)
+            for (int k = A.rp[i]; k < A.rp[i + 1]; k++) S.v[k] = -w * L.dinv[i] * A.v[k] + (A.ci[k] == i ? 1.0 : 0.0);
+        L.P = csr_mul(S, Pt);
+        L.R = csr_transpose(L.P);
+        HCsr AP = csr_mul(A, L.P);
+        A = csr_mul(L.R, AP);
+        lv.push_back(std::move(L));
+    }
+    if (lv.empty() || A.nrow > 2000) return false;  // coarsening stalled
+    // dense inverse of the coarsest matrix
+    nc = A.nrow;
+    std::vector<double> M((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; i++)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) M[(size_t)i * nc + A.ci[k]] += A.v[k];
+    cinv.assign((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; i++) cinv[(size_t)i * nc + i] = 1.0;
+    double amax = 0;
+    for (double a : M) amax = std::max(amax, std::fabs(a));
+    for (int c = 0; c < nc; c++) {
+        int pr = c;
+        for (int r = c + 1; r < nc; r++)
+            if (std::fabs(M[(size_t)r * nc + c]) > std::fabs(M[(size_t)pr * nc + c])) pr = r;
+        if (!(std::fabs(M[(size_t)pr * nc + c]) > 1e-14 * amax)) return false;
+        if (pr != c)
+            for (int j = 0; j < nc; j++) {
+                std::swap(M[(size_t)pr * nc + j], M[(size_t)c * nc + j]);
+                std::swap(cinv[(size_t)pr * nc + j], cinv[(size_t)c * nc + j]);
+            }
+        const double d = 1.0 / M[(size_t)c * nc + c];
+        for (int j = 0; j < nc; j++) {
+            M[(size_t)c * nc + j] *= d;
+            cinv[(size_t)c * nc + j] *= d;
+        }
+        for (int r = 0; r < nc; r++) {
+            if (r == c) continue;
+            const double f = M[(size_t)r * nc + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < nc; j++) {
+                M[(size_t)r * nc + j] -= f * M[(size_t)c * nc + j];
+                cinv[(size_t)r * nc + j] -= f * cinv[(size_t)c * nc + j];
+            }
+        }
+    }
+    for (double a : cinv)
+        if (!std::isfinite(a)) return false;
+    return true;
+}
+
+// ------------------------------------------------------------ V-cycle kernels
+constexpr int AL = 4;     // lanes per row in the V-cycle kernels
+constexpr int ANT = 256;  // threads per workgroup
+
+__device__ inline bool amg_done(const int *ki) { return ki[KI_DONE] != 0; }
+
+template <typename T>
+__device__ inline T arow(int rb, int re, const int *__restrict__ ci, const T *__restrict__ v, const T *__restrict__ x) {
+    T s = 0;
+    for (int k = rb + (int)(threadIdx.x % AL); k < re; k += AL) s += v[k] * x[ci[k]];
+#pragma unroll
+    for (int o = AL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+// r = b - A (omega D^-1 b)
+template <typename T>
+__global__ void __launch_bounds__(ANT) k_amg_resid(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                   const T *__restrict__ v, const T *__restrict__ dinv, T omega,
+                                                   const T *__restrict__ b, T *__restrict__ r, const int *ki) {
+    if (amg_done(ki)) return;
+    const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
+    const bool in = i < n;
+    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
+    T s = 0;
+    for (int k = rb + (int)(threadIdx.x % AL); k < re; k += AL) {
+        const int j = ci[k];
+        s += v[k] * (omega * (dinv[j] * b[j]));
+    }
+#pragma unroll
+    for (int o = AL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (in && threadIdx.x % AL == 0) r[i] = b[i] - s;
+}
+
+// b_c = R r
+template <typename T>
+__global__ void __launch_bounds__(ANT) k_amg_restrict(int nc, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                      const T *__restrict__ v, const T *__restrict__ r,
+                                                      T *__restrict__ bc, const int *ki) {
+    if (amg_done(ki)) return;
+    const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
+    const bool in = i < nc;
+    const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, r);
+    if (in && threadIdx.x % AL == 0) bc[i] = s;
+}
+
+// coarsest level, one workgroup: b_c = R r, x_c = C^-1 b_c
+template <typename T>
+__global__ void __launch_bounds__(1024) k_amg_coarse(int nc, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const T *__restrict__ v, const T *__restrict__ r,
+                                                     const T *__restrict__ cinv, T *__restrict__ xc, const int *ki) {
+    if (amg_done(ki)) return;
+    extern __shared__ unsigned char amg_sh[];
+    T *bs = reinterpret_cast<T *>(amg_sh);
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        T s = 0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * r[ci[k]];
+        bs[i] = s;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        const T *row = cinv + (size_t)i * nc;
+        T s = 0;
+        for (int j = 0; j < nc; j++) s += row[j] * bs[j];
+        xc[i] = s;
+    }
+}
+
+// x = omega D^-1 b + P x_c
+template <typename T>
+__global__ void __launch_bounds__(ANT) k_amg_prolong(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const T *__restrict__ v, const T *__restrict__ dinv, T omega,
+                                                     const T *__restrict__ b, const T *__restrict__ xc,
+                                                     T *__restrict__ x, const int *ki) {
+    if (amg_done(ki)) return;
+    const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
+    const bool in = i < n;
+    const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, xc);
+    if (in && threadIdx.x % AL == 0) x[i] = omega * (dinv[i] * b[i]) + s;
+}
+
+// y = x + omega D^-1 (b - A x)
+template <typename T>
+__global__ void __launch_bounds__(ANT) k_amg_post(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const T *__restrict__ v, const T *__restrict__ dinv, T omega,
+                                                  const T *__restrict__ b, const T *__restrict__ x,
+                                                  T *__restrict__ y, const int *ki) {
+    if (amg_done(ki)) return;
+    const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
+    const bool in = i < n;
+    const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, x);
+    if (in && threadIdx.x % AL == 0) y[i] = x[i] + omega * (dinv[i] * (b[i] - s));
+}
+
+// The hierarchy on the device.  apply(b, y): y = V(b); b and y have the
+// fine size, y must differ from b.
+template <typename T>
+class AmgPrecond {
+  public:
+    bool ready = false;
+    int nlev = 0, ncoarse = 0;
+    T omega = T(0.6);
+
+    bool build(int n, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<double> &v,
+               hipStream_t st) {
+        HCsr A;
+        A.nrow = A.ncol = n;
+        A.rp = rp;
+        A.ci = ci;
+        A.v = v;
+        std::vector<AmgHostLevel> hl;
+        std::vector<double> cinv;
+        ready = false;
+        if (!sa_build(A, hl, cinv, ncoarse)) return false;
+        nlev = (int)hl.size();
+        lv_.clear();
+        lv_.resize(nlev);
+        for (int l = 0; l < nlev; l++) {
+            Dev &d = lv_[l];
+            d.n = hl[l].A.nrow;
+            d.nc = hl[l].P.ncol;
+            upi(d.Arp, hl[l].A.rp, st);
+            upi(d.Aci, hl[l].A.ci, st);
+            upt(d.Av, hl[l].A.v, st);
+            upt(d.dinv, hl[l].dinv, st);
+            upi(d.Prp, hl[l].P.rp, st);
+            upi(d.Pci, hl[l].P.ci, st);
+            upt(d.Pv, hl[l].P.v, st);
+            upi(d.Rrp, hl[l].R.rp, st);
+            upi(d.Rci, hl[l].R.ci, st);
+            upt(d.Rv, hl[l].R.v, st);
+            d.r.alloc(d.n);
+            d.x.alloc(d.n);
+            d.bnext.alloc(d.nc);
+            d.xnext.alloc(d.nc);
+        }
+        upt(cinv_, cinv, st);
+        if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("amg upload failed");
+        ready = true;
+        return true;
+    }
+
+    // y = V(b) on the fine level; `ki` is the solve's flag array
+    void apply(const T *b, T *y, const int *ki, hipStream_t st) { cycle(0, b, y, ki, st); }
+
+    long bytes_per_cycle() const {
+        long by = 0;
+        for (const Dev &d : lv_) by += 3L * (d.Arp.n * 4 + d.Av.n * (4 + sizeof(T))) + (d.Prp.n + d.Rrp.n) * 4;
+        return by;
+    }
+
+  private:
+    struct Dev {
+        int n = 0, nc = 0;
+        KBuf<int> Arp, Aci, Prp, Pci, Rrp, Rci;
+        KBuf<T> Av, dinv, Pv, Rv, r, x, bnext, xnext;
+    };
+    std::vector<Dev> lv_;
+    KBuf<T> cinv_;
+
+    static void upi(KBuf<int> &d, const std::vector<int> &h, hipStream_t st) {
+        d.alloc(h.size());
+        if (!h.empty() && hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+            throw std::runtime_error("hipMemcpyAsync failed");
+    }
+    static void upt(KBuf<T> &d, const std::vector<double> &h, hipStream_t st) {
+        std::vector<T> t(h.begin(), h.end());
+        d.alloc(t.size());
+        if (!t.empty() && hipMemcpyAsync(d.p, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+            throw std::runtime_error("hipMemcpyAsync failed");
+        if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("hipStreamSynchronize failed");
+    }
+    static int grid(int rows) { return std::max(1, (rows * AL + ANT - 1) / ANT); }
+
+    // y = V(b) at level l (y, b of that level's size)
+    void cycle(int l, const T *b, T *y, const int *ki, hipStream_t st) {
+        Dev &d = lv_[l];
+        k_amg_resid<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.r.p, ki);
+        if (l + 1 < nlev) {
+            k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
+            cycle(l + 1, d.bnext.p, d.xnext.p, ki, st);
+        } else {
+            k_amg_coarse<T><<<1, 1024, ncoarse * sizeof(T), st>>>(ncoarse, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, cinv_.p,
+                                                                   d.xnext.p, ki);
+        }
+        k_amg_prolong<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b, d.xnext.p,
+                                                    d.x.p, ki);
+        k_amg_post<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.x.p, y, ki);
+    }
+};
+
+}  // namespace hfh
+
+namespace hfh {
+
+// ------------------------------------------- BiCGStab with the V-cycle
+// Right preconditioning x = M^-1 u with M^-1 one V-cycle:
+//   A(it):  it = 0: rho = rh.r, p = r.  it > 0: omega = t.s / t.t,
+//           x += alpha ph + omega sh, r = s - omega t,
+//           rho = rh.s - omega rh.t, beta = (rho / rho_old)(alpha / omega),
+//           p = r + beta (p - omega v)                    (pointwise)
+//   ph = V(p)
+//   B(it):  stopping test on r.r; v = A ph; partial rh.v
+//   C(it):  alpha = rho / rh.v; s = r - alpha v            (pointwise)
+//   sh = V(s)
+//   D(it):  t = A sh; partials t.s, t.t, rh.s, rh.t
+// Rows of the SpMV launches run on L lanes; the pointwise launches use the
+// same rows per workgroup so the partial arrays line up.
+
+template <typename T, int L>
+__global__ void __launch_bounds__(KNT) k_abcg_a(int n, int it, T *__restrict__ x, T *__restrict__ r,
+                                                const T *__restrict__ s, const T *__restrict__ t,
+                                                const T *__restrict__ ph, const T *__restrict__ sh,
+                                                T *__restrict__ p, const T *__restrict__ v, T *__restrict__ part,
+                                                int nb, T *ks, int *ki) {
+    const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
+    const bool own = (int)threadIdx.x < KNT / L && i < n;
+    if (it == 0) {
+        const int sl[1] = {KP_RZ};
+        __shared__ T pv[2];
+        reduce_partials<T, 1>(part, nb, sl, ki, pv);
+        const T ri = own ? r[i] : T(0);
+        kblock_sync();
+        if (pv[1] != T(0)) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO] = pv[0];
+        if (own) p[i] = ri;
+        return;
+    }
+    const int sl[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
+    __shared__ T pv[5];
+    reduce_partials<T, 4>(part, nb, sl, ki, pv, KI_BRK + ((it + 1) & 1));
+    T xi = 0, si = 0, ti = 0, phi = 0, shi = 0, pi = 0, vi = 0;
+    if (own) {
+        xi = x[i];
+        si = s[i];
+        ti = t[i];
+        phi = ph[i];
+        shi = sh[i];
+        pi = p[i];
+        vi = v[i];
+    }
+    kblock_sync();
+    if (pv[4] != T(0)) return;
+    const int o = (it + 1) & 1;
+    const T omega = pv[1] > T(0) ? pv[0] / pv[1] : T(0);
+    const T alpha = ks[KS_ALPHA + o], rho_old = ks[KS_RHO + o];
+    const T rho = pv[2] - omega * pv[3];
+    const bool breakdown = rho == T(0) || rho_old == T(0) || omega == T(0);
+    const T beta = breakdown ? T(0) : (rho / rho_old) * (alpha / omega);
+    T acc[1] = {0};
+    T ri = 0;
+    if (own) {
+        x[i] = xi + alpha * phi + omega * shi;
+        ri = si - omega * ti;
+        r[i] = ri;
+    }
+    if (breakdown) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ki[KI_ITERS] = it;
+            ki[KI_CONV] = 0;
+            ki[KI_BRK + (it & 1)] = 1;
+        }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
+    if (own) {
+        p[i] = ri + beta * (pi - omega * vi);
+        acc[0] = ri * ri;
+    }
+    const int so[1] = {KP_RR};
+    write_partials<T, 1>(acc, part, nb, so);
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol, T atol,
+                                                const int *__restrict__ rp, const int *__restrict__ ci,
+                                                const T *__restrict__ val, const T *__restrict__ ph,
+                                                const T *__restrict__ rh, T *__restrict__ v, T *__restrict__ part,
+                                                int nb, T *ks, int *ki) {
+    const int sl[2] = {KP_RR, KP_BB};
+    __shared__ T pv[3];
+    reduce_partials<T, 2>(part, nb, sl, ki, pv, KI_BRK + (it & 1));
+    const int i = krow<L>();
+    const bool in = i < n;
+    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
+    const T *const vv[1] = {ph};
+    T sv[1];
+    row_sums<T, L, 1>(rb, re, ci, val, vv, sv);
+    const T rhi = in ? rh[i] : T(0);
+    kblock_sync();
+    if (pv[2] != T(0)) return;
+    if (krylov_stop<T>(it == 0, it, maxit, pv[0], pv[1], rtol, atol, false, ks, ki)) return;
+    T acc[1] = {0};
+    if (in && klane<L>() == 0) {
+        v[i] = sv[0];
+        acc[0] = rhi * sv[0];
+    }
+    const int so[1] = {KP_RV};
+    write_partials<T, 1>(acc, part, nb, so);
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(KNT) k_abcg_c(int n, int it, const T *__restrict__ r, const T *__restrict__ v,
+                                                T *__restrict__ s, T *__restrict__ part, int nb, T *ks,
+                                                const int *ki) {
+    const int sl[1] = {KP_RV};
+    __shared__ T pv[2];
+    reduce_partials<T, 1>(part, nb, sl, ki, pv, KI_BRK + (it & 1));
+    const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
+    const bool own = (int)threadIdx.x < KNT / L && i < n;
+    const T ri = own ? r[i] : T(0), vi = own ? v[i] : T(0);
+    kblock_sync();
+    if (pv[1] != T(0)) return;
+    const T rho = ks[KS_RHO + (it & 1)];
+    const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
+    if (own) s[i] = ri - alpha * vi;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(KNT) k_abcg_d(int n, int it, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                const T *__restrict__ val, const T *__restrict__ sh,
+                                                const T *__restrict__ s, const T *__restrict__ rh,
+                                                T *__restrict__ t, T *__restrict__ part, int nb, const int *ki) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = ki[KI_DONE] + ki[KI_BRK + (it & 1)];
+    const int i = krow<L>();
+    const bool in = i < n;
+    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
+    const T *const vv[1] = {sh};
+    T sv[1];
+    row_sums<T, L, 1>(rb, re, ci, val, vv, sv);
+    const T si = in ? s[i] : T(0), rhi = in ? rh[i] : T(0);
+    kblock_sync();
+    if (s_done) return;
+    T acc[4] = {0, 0, 0, 0};
+    if (in && klane<L>() == 0) {
+        const T ti = sv[0];
+        t[i] = ti;
+        acc[0] = ti * si;
+        acc[1] = ti * ti;
+        acc[2] = rhi * si;
+        acc[3] = rhi * ti;
+    }
+    const int so[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
+    write_partials<T, 4>(acc, part, nb, so);
+}
+
+}  // namespace hfh

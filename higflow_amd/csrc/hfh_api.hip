@@ -1,5 +1,6 @@
 // C ABI of libhigflow_hip.so (declarations in include/higflow_hip.h).
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <string>
 
@@ -187,7 +188,8 @@ int hfh_slv_assemble_matrix(hfh_slv *s) {
             }
             rp.push_back((int)ci.size());
         }
-        s->sys.set_matrix(rp, ci, v, s->stream);
+        const char *ea = std::getenv("HFH_SLV_AMG");  // opt-in V-cycle preconditioning
+        s->sys.set_matrix(rp, ci, v, s->stream, ea && std::strcmp(ea, "1") == 0);
         s->db.alloc(s->n);
         s->dx.alloc(s->n);
         s->dirty = false;

@@ -182,7 +182,7 @@ void Flow<D>::finalize_layout() {
         upload_group(dGrad_, gGrad_, 0, 0, true);
         dGrad_.y.upload(y.data(), y.size(), st_);
     }
-    psys_.set_matrix(Arp_, Aci_, Av_, st_);
+    psys_.set_matrix(Arp_, Aci_, Av_, st_, true);
     if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
         for (int d = 0; d < D; d++) {
             hsys_[d].set_matrix(Hrp_[d], Hci_[d], Hv_[d], st_);

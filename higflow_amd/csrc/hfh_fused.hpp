@@ -1,0 +1,205 @@
+// Fused Krylov solver over one CSR matrix: Jacobi-preconditioned PCG and
+// BiCGStab in two launches per iteration (hfh_krylov.hpp), or BiCGStab with
+// a smoothed-aggregation V-cycle as right preconditioner (hfh_amg.hpp).
+#pragma once
+#include <memory>
+
+#include "hfh_amg.hpp"
+#include "hfh_krylov.hpp"
+
+namespace hfh {
+
+// One precision's fused solver over a CSR matrix whose structure (rp, ci)
+// the owner keeps on the device.
+template <typename T>
+class FusedKrylov {
+  public:
+    int n = 0, nb = 0, lanes = 1;
+    const int *rp = nullptr, *ci = nullptr;
+    KBuf<T> val, vhat, dinv;  // A, A D^-1, D^-1
+    KBuf<T> r, rh, z, p, p2, q, v, v2, s, t, part, ks;
+    KBuf<int> ki;
+    int pred = 0;  // iterations of the last solve (first chunk)
+    // BiCGStab right-preconditioned by a smoothed-aggregation V-cycle
+    // (requested before set(); used when the hierarchy builds)
+    bool want_amg = false;
+    AmgPrecond<T> amg;
+    KBuf<T> ph, sh;
+
+    FusedKrylov() = default;
+    FusedKrylov(const FusedKrylov &) = delete;
+    FusedKrylov &operator=(const FusedKrylov &) = delete;
+
+    void set(int n_, const int *drp, const int *dci, const std::vector<int> &hrp, const std::vector<int> &hci,
+             const std::vector<double> &hv, hipStream_t st) {
+        n = n_;
+        rp = drp;
+        ci = dci;
+        std::vector<double> hd(n, 1.0);
+        for (int i = 0; i < n; i++)
+            for (int k = hrp[i]; k < hrp[i + 1]; k++)
+                if (hci[k] == i && hv[k] != 0.0) hd[i] = 1.0 / hv[k];
+        std::vector<T> tv(hv.size()), th(hv.size()), td(n);
+        for (size_t k = 0; k < hv.size(); k++) {
+            tv[k] = (T)hv[k];
+            th[k] = (T)(hv[k] * hd[hci[k]]);
+        }
+        for (int i = 0; i < n; i++) td[i] = (T)hd[i];
+        up(val, tv, st);
+        up(vhat, th, st);
+        up(dinv, td, st);
+        for (KBuf<T> *b : {&r, &rh, &z, &p, &p2, &q, &v, &v2, &s, &t}) b->alloc(n);
+        ks.alloc(KS_N);
+        ki.alloc(KI_N);
+        pred = 0;
+        tune(st);
+        amg.ready = false;
+        if (want_amg && amg.build(n, hrp, hci, hv, st)) {
+            ph.alloc(n);
+            sh.alloc(n);
+        }
+    }
+    bool amg_on() const { return amg.ready; }
+
+    // r = b - A x and the flags; x is the initial guess
+    template <typename TB>
+    void init(bool cg, const TB *b, T *x, hipStream_t st) {
+        switch (lanes) {
+        case 1: launch_init<TB, 1>(cg, b, x, st); break;
+        case 2: launch_init<TB, 2>(cg, b, x, st); break;
+        case 4: launch_init<TB, 4>(cg, b, x, st); break;
+        default: launch_init<TB, 8>(cg, b, x, st); break;
+        }
+    }
+    // iterations [it0, it0 + count)
+    void iterate(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
+        switch (lanes) {
+        case 1: launch_iter<1>(cg, it0, count, maxit, rtol, atol, x, st); break;
+        case 2: launch_iter<2>(cg, it0, count, maxit, rtol, atol, x, st); break;
+        case 4: launch_iter<4>(cg, it0, count, maxit, rtol, atol, x, st); break;
+        default: launch_iter<8>(cg, it0, count, maxit, rtol, atol, x, st); break;
+        }
+    }
+
+    // Times `reps` launches of BiCGStab launch B, t = (A D^-1)(r - alpha v)
+    // (the solve's SpMV launch shape), on the current vectors with the flags
+    // cleared; call between solves only.  Returns the average ms and the
+    // algorithmic bytes of one launch: CSR structure and values, r, v and rh
+    // read once, s and t written once, the partials.
+    float probe(int reps, long *bytes, hipStream_t st) {
+        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+        float ms = 0;
+        switch (lanes) {
+        case 1: ms = time_b<1>(reps, st); break;
+        case 2: ms = time_b<2>(reps, st); break;
+        case 4: ms = time_b<4>(reps, st); break;
+        default: ms = time_b<8>(reps, st); break;
+        }
+        long nnz = 0;
+        int last = 0;
+        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
+        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 5 * sizeof(T) +
+                 (long)nb * 6 * sizeof(T);
+        return ms;
+    }
+
+  private:
+    static void up(KBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
+        d.alloc(h.size());
+        if (!h.empty() &&
+            hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+            throw std::runtime_error("hipMemcpyAsync failed");
+    }
+    template <int L>
+    int blocks() const {
+        return std::max(1, (n + KNT / L - 1) / (KNT / L));
+    }
+
+    template <typename TB, int L>
+    void launch_init(bool cg, const TB *b, T *x, hipStream_t st) {
+        k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, rp, ci, val.p, x, b, dinv.p, r.p, z.p, rh.p, part.p, nb,
+                                              ki.p);
+    }
+    template <int L>
+    void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
+        if (!cg && amg.ready) {
+            for (int it = it0; it < it0 + count; it++) {
+                k_abcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, s.p, t.p, ph.p, sh.p, p.p, v.p, part.p, nb, ks.p,
+                                                   ki.p);
+                amg.apply(p.p, ph.p, ki.p, st);
+                k_abcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, ph.p, rh.p, v.p, part.p,
+                                                   nb, ks.p, ki.p);
+                k_abcg_c<T, L><<<nb, KNT, 0, st>>>(n, it, r.p, v.p, s.p, part.p, nb, ks.p, ki.p);
+                amg.apply(s.p, sh.p, ki.p, st);
+                k_abcg_d<T, L><<<nb, KNT, 0, st>>>(n, it, rp, ci, val.p, sh.p, s.p, rh.p, t.p, part.p, nb, ki.p);
+            }
+            return;
+        }
+        for (int it = it0; it < it0 + count; it++) {
+            T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
+            if (cg) {
+                k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, z.p, pold, pnew, q.p,
+                                                     part.p, nb, ks.p, ki.p);
+                k_kcg_update<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb, ks.p, ki.p);
+            } else {
+                T *vold = (it & 1) ? v.p : v2.p, *vnew = (it & 1) ? v2.p : v.p;
+                k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, vhat.p, dinv.p, x, r.p, rh.p, s.p,
+                                                   t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p);
+                k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rp, ci, vhat.p, r.p, rh.p, vnew, s.p, t.p, part.p,
+                                                   nb, ks.p, ki.p);
+            }
+        }
+    }
+    template <int L>
+    float time_b(int reps, hipStream_t st) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            throw std::runtime_error("hipEventCreate failed");
+        (void)hipEventRecord(a, st);
+        for (int rep = 0; rep < reps; rep++)
+            k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, 0, 1 << 30, rp, ci, vhat.p, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
+                                               ks.p, ki.p);
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return reps > 0 ? ms / reps : 0.0f;
+    }
+    template <int L>
+    float time_lanes(hipStream_t st) {
+        lanes = L;
+        nb = blocks<L>();
+        part.alloc((size_t)KP_N * nb);
+        // launch B on zero vectors with the flags cleared: reads and writes
+        // only buffers of this solver
+        for (KBuf<T> *b : {&r, &rh, &v}) k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, b->p, T(0));
+        k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
+        k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
+        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+        time_b<L>(2, st);
+        return time_b<L>(20, st);
+    }
+    // picks the lanes per row with the fastest SpMV launch
+    void tune(hipStream_t st) {
+        const float tl[4] = {time_lanes<1>(st), time_lanes<2>(st), time_lanes<4>(st), time_lanes<8>(st)};
+        const int ll[4] = {1, 2, 4, 8};
+        int best = 1;
+        float tb = 1e30f;
+        for (int k = 0; k < 4; k++)
+            if (tl[k] < tb) {
+                tb = tl[k];
+                best = ll[k];
+            }
+        if (const char *e = std::getenv("HFH_KRYLOV_LANES")) {
+            int f = std::atoi(e);
+            if (f == 1 || f == 2 || f == 4 || f == 8) best = f;
+        }
+        lanes = best;
+        nb = best == 1 ? blocks<1>() : best == 2 ? blocks<2>() : best == 4 ? blocks<4>() : blocks<8>();
+        part.alloc((size_t)KP_N * nb);
+    }
+};
+
+}  // namespace hfh

@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "hfh_kernels.hpp"
-#include "hfh_krylov.hpp"
+#include "hfh_fused.hpp"
 
 #define HFH_CHECK(x)                                                                        \
     do {                                                                                    \
@@ -388,8 +388,10 @@ class DeviceSystem {
         if (hpin_) (void)hipHostFree(hpin_);
     }
 
+    // `amg`: precondition the fused BiCGStab with a smoothed-aggregation
+    // V-cycle (HFH_AMG=0 turns it off)
     void set_matrix(const std::vector<int> &hrp, const std::vector<int> &hci, const std::vector<double> &hv,
-                    hipStream_t st) {
+                    hipStream_t st, bool amg = false) {
         n = (int)hrp.size() - 1;
         nnz = (int)hci.size();
         rp.upload(hrp.data(), hrp.size(), st);
@@ -418,6 +420,10 @@ class DeviceSystem {
         if (!hpin_) HFH_CHECK(hipHostMalloc((void **)&hpin_, 512, hipHostMallocDefault));
         if ((int)xbak.n < n) xbak.alloc(n);
         if (const char *er = std::getenv("HFH_RESTARTS")) max_restarts = std::max(0, std::atoi(er));
+        {
+            const char *ea = std::getenv("HFH_AMG");
+            k64.want_amg = k32.want_amg = amg && !(ea && std::strcmp(ea, "0") == 0);
+        }
         k64.set(n, rp.p, ci.p, hrp, hci, hv, st);
         k32.set(n, rp.p, ci.p, hrp, hci, hv, st);
         // HFH_KRYLOV=classic selects the one-kernel-per-operation iterations

@@ -83,17 +83,16 @@ def _contraction_pressure(levels, h0, length):
     return sp.csr_matrix((v, ci, rp))
 
 
-@pytest.mark.parametrize("coarse", ["0", "1"])
-def test_fused_bicgstab_on_contraction_pressure(coarse, monkeypatch):
-    # fused BiCGStab (with and without the workgroup-block coarse
-    # correction) on a real nonsymmetric AMR pressure operator, against a
-    # direct solve
+@pytest.mark.parametrize("amg", ["0", "1"])
+def test_fused_bicgstab_on_contraction_pressure(amg, monkeypatch):
+    # fused BiCGStab, Jacobi or smoothed-aggregation V-cycle preconditioned,
+    # on a real nonsymmetric AMR pressure operator, against a direct solve
     import scipy.sparse.linalg as sla
     A = _contraction_pressure(2, 0.25, 8.0)
     n = A.shape[0]
     x0 = np.random.default_rng(5).normal(size=n)
     b = A @ x0
-    monkeypatch.setenv("HFH_COARSE", coarse)
+    monkeypatch.setenv("HFH_SLV_AMG", amg)
     s = native.LinearSolver(0, n)
     s.set_tolerances(1e-12, 0.0, 20000)
     for i in range(n):
@@ -103,7 +102,7 @@ def test_fused_bicgstab_on_contraction_pressure(coarse, monkeypatch):
     s.solve()
     x = s.get_x()
     conv, iters, resid = s.stats()[:3]
-    print("coarse", coarse, "n", n, "iters", iters, "resid", resid)
+    print("amg", amg, "n", n, "iters", iters, "resid", resid)
     xd = sla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-11
     assert np.abs(x - xd).max() / np.abs(xd).max() < 1e-7
