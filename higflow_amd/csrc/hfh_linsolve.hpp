@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -379,6 +380,9 @@ class DeviceSystem {
     bool fused = true;
     int fp64_solves_left = 0;  // mixed: solves to run in FP64 after an FP32 failure
     int max_restarts = 1;      // fused64: restarts on the true residual
+    double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
+    int since_probe_ = 0;
+    bool force32_ = false;  // HFH_MIXED_FP32=1: always refine with FP32 solves
     DevBuf<double> xbak;
 
     DeviceSystem() = default;
@@ -423,6 +427,12 @@ class DeviceSystem {
         {
             const char *ea = std::getenv("HFH_AMG");
             k64.want_amg = k32.want_amg = amg && !(ea && std::strcmp(ea, "0") == 0);
+        }
+        t_mixed_ = t_fp64_ = -1;
+        since_probe_ = 0;
+        {
+            const char *em = std::getenv("HFH_MIXED_FP32");
+            force32_ = em && std::strcmp(em, "1") == 0;
         }
         k64.set(n, rp.p, ci.p, hrp, hci, hv, st);
         k32.set(n, rp.p, ci.p, hrp, hci, hv, st);
@@ -503,8 +513,47 @@ class DeviceSystem {
     // BiCGStab) of A e = r from e = 0 to inner_rtol.  Each inner solve runs
     // to its own tolerance (a truncated BiCGStab correction can raise the
     // error).
+    // Mixed form: per system, FP32 refinement or the FP64 solve, whichever
+    // took less wall time on its last solves (both are timed on the host
+    // clock; each solve ends with a host synchronisation).  The form not in
+    // use is re-timed every 50 solves.
     SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                            int maxit, hipStream_t st) {
+        bool use32;
+        if (force32_)
+            use32 = true;
+        else if (t_fp64_ < 0)
+            use32 = false;
+        else if (t_mixed_ < 0)
+            use32 = true;
+        else {
+            use32 = t_mixed_ < t_fp64_;
+            if (++since_probe_ >= 50) {
+                use32 = !use32;
+                since_probe_ = 0;
+            }
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        SolveStats S;
+        if (use32) {
+            S = mixed_refine(spd, b, x, rtol, atol, inner_rtol, maxit, st);
+        } else {
+            S = fused64(spd, b, x, rtol, atol, maxit, st);
+            S.fallback = 2;
+        }
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (use32 && S.fallback == 0)
+            t_mixed_ = t_mixed_ < 0 ? dt : 0.5 * (t_mixed_ + dt);
+        else if (use32)
+            t_mixed_ = 1e30;  // FP32 failed on this system: keep FP64
+        else
+            t_fp64_ = t_fp64_ < 0 ? dt : 0.5 * (t_fp64_ + dt);
+        return S;
+    }
+
+    // FP64 defect correction around FP32 solves (see fused_mixed)
+    SolveStats mixed_refine(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
+                            int maxit, hipStream_t st) {
         if (fp64_solves_left > 0) {
             // the FP32 corrections failed on this system recently
             fp64_solves_left--;

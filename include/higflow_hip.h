@@ -139,8 +139,9 @@ int hfh_last_solve(hfh_solver *s, int *iters, double *resid, int *outer);
 /* the last solve of a system (0 pressure, 1+d semi-implicit velocity
  * component d), as the solver's SolverIterations / SolverResidual /
  * converged (higtree/src/solver.h:80-82): info = {iterations, refinement
- * steps, finished-in-FP64 flag (mixed), true-residual restarts}; returns
- * 1 converged, 0 not converged, <0 error */
+ * steps, mixed form's FP64 use (0 FP32 refinement, 1 FP32 failed and the
+ * solve finished in FP64, 2 FP64 chosen as the faster form), true-residual
+ * restarts}; returns 1 converged, 0 not converged, <0 error */
 int hfh_solve_stats(hfh_solver *s, int which, int *info, double *resid);
 
 /* sd_get_stencil / sfd_get_stencil on the library's host mesh: domain 0

@@ -96,7 +96,9 @@ def test_viscoelastic_cavity_parity(model, convec):
     compare(case, 5)
 
 
-def test_mixed_precision_error_bound():
+def test_mixed_precision_error_bound(monkeypatch):
+    # FP32 refinement on every solve (not only where it is the faster form)
+    monkeypatch.setenv("HFH_MIXED_FP32", "1")
     case = cases.cavity(64, Re=100.0, **TIGHT)
     h64 = native.Flow(case)
     hmx = native.Flow(case, precision=1, inner_rtol=1e-5)
@@ -170,8 +172,11 @@ def test_fused_and_classic_krylov_agree(which, monkeypatch):
     _agree(hf, hc, case["dim"], case["params"]["flowtype"] == 3, 1e-10)
 
 
-def test_mixed_precision_amr_contraction():
-    # nonsymmetric systems: FP32 BiCGStab corrections inside FP64 refinement
+@pytest.mark.parametrize("force32", ["0", "1"])
+def test_mixed_precision_amr_contraction(force32, monkeypatch):
+    # nonsymmetric systems: FP32 BiCGStab corrections inside FP64 refinement,
+    # forced on every solve or chosen by timing
+    monkeypatch.setenv("HFH_MIXED_FP32", force32)
     case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, Re=0.01, Wi=1.0, beta=1.0 / 9.0,
                              dt=1e-4, rtol=1e-10, atol=1e-10, maxit=50000)
     h64 = native.Flow(case)
