@@ -303,7 +303,8 @@ __global__ void __launch_bounds__(ANT) k_amg_restrict(int nc, const int *__restr
     if (in && threadIdx.x % AL == 0) bc[i] = s;
 }
 
-// coarsest level, one workgroup: b_c = R r, x_c = C^-1 b_c
+// coarsest level, one workgroup: b_c = R r, x_c = C^-1 b_c (a wave per
+// row: lanes stride the row, then a butterfly)
 template <typename T>
 __global__ void __launch_bounds__(1024) k_amg_coarse(int nc, const int *__restrict__ rp, const int *__restrict__ ci,
                                                      const T *__restrict__ v, const T *__restrict__ r,
@@ -311,17 +312,20 @@ __global__ void __launch_bounds__(1024) k_amg_coarse(int nc, const int *__restri
     if (amg_done(ki)) return;
     extern __shared__ unsigned char amg_sh[];
     T *bs = reinterpret_cast<T *>(amg_sh);
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = w; i < nc; i += nw) {
         T s = 0;
-        for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * r[ci[k]];
-        bs[i] = s;
+        for (int k = rp[i] + lane; k < rp[i + 1]; k += 64) s += v[k] * r[ci[k]];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) bs[i] = s;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    for (int i = w; i < nc; i += nw) {
         const T *row = cinv + (size_t)i * nc;
         T s = 0;
-        for (int j = 0; j < nc; j++) s += row[j] * bs[j];
-        xc[i] = s;
+        for (int j = lane; j < nc; j += 64) s += row[j] * bs[j];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) xc[i] = s;
     }
 }
 
@@ -472,22 +476,23 @@ __global__ void __launch_bounds__(KNT) k_abcg_a(int n, int it, T *__restrict__ x
                                                 const T *__restrict__ ph, const T *__restrict__ sh,
                                                 T *__restrict__ p, const T *__restrict__ v, T *__restrict__ part,
                                                 int nb, T *ks, int *ki) {
+    const KCtx c = kctx<T>(KSeg(), nb, ks, ki);
     const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
     const bool own = (int)threadIdx.x < KNT / L && i < n;
     if (it == 0) {
         const int sl[1] = {KP_RZ};
         __shared__ T pv[2];
-        reduce_partials<T, 1>(part, nb, sl, ki, pv);
+        reduce_partials<T, 1>(part, nb, c, sl, ki, pv);
         const T ri = own ? r[i] : T(0);
         kblock_sync();
         if (pv[1] != T(0)) return;
-        if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO] = pv[0];
+        if (c.lead && threadIdx.x == 0) ks[KS_RHO] = pv[0];
         if (own) p[i] = ri;
         return;
     }
     const int sl[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
     __shared__ T pv[5];
-    reduce_partials<T, 4>(part, nb, sl, ki, pv, KI_BRK + ((it + 1) & 1));
+    reduce_partials<T, 4>(part, nb, c, sl, ki, pv, KI_BRK + ((it + 1) & 1));
     T xi = 0, si = 0, ti = 0, phi = 0, shi = 0, pi = 0, vi = 0;
     if (own) {
         xi = x[i];
@@ -514,14 +519,14 @@ __global__ void __launch_bounds__(KNT) k_abcg_a(int n, int it, T *__restrict__ x
         r[i] = ri;
     }
     if (breakdown) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (c.lead && threadIdx.x == 0) {
             ki[KI_ITERS] = it;
             ki[KI_CONV] = 0;
             ki[KI_BRK + (it & 1)] = 1;
         }
         return;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
+    if (c.lead && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
     if (own) {
         p[i] = ri + beta * (pi - omega * vi);
         acc[0] = ri * ri;
@@ -536,9 +541,10 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
                                                 const T *__restrict__ val, const T *__restrict__ ph,
                                                 const T *__restrict__ rh, T *__restrict__ v, T *__restrict__ part,
                                                 int nb, T *ks, int *ki) {
+    const KCtx c = kctx<T>(KSeg(), nb, ks, ki);
     const int sl[2] = {KP_RR, KP_BB};
     __shared__ T pv[3];
-    reduce_partials<T, 2>(part, nb, sl, ki, pv, KI_BRK + (it & 1));
+    reduce_partials<T, 2>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
     const int i = krow<L>();
     const bool in = i < n;
     const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
@@ -548,7 +554,7 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
     const T rhi = in ? rh[i] : T(0);
     kblock_sync();
     if (pv[2] != T(0)) return;
-    if (krylov_stop<T>(it == 0, it, maxit, pv[0], pv[1], rtol, atol, false, ks, ki)) return;
+    if (krylov_stop<T>(c.lead, it == 0, it, maxit, pv[0], pv[1], rtol, atol, false, ks, ki)) return;
     T acc[1] = {0};
     if (in && klane<L>() == 0) {
         v[i] = sv[0];
@@ -560,11 +566,11 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
 
 template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_abcg_c(int n, int it, const T *__restrict__ r, const T *__restrict__ v,
-                                                T *__restrict__ s, T *__restrict__ part, int nb, T *ks,
-                                                const int *ki) {
+                                                T *__restrict__ s, T *__restrict__ part, int nb, T *ks, int *ki) {
+    const KCtx c = kctx<T>(KSeg(), nb, ks, ki);
     const int sl[1] = {KP_RV};
     __shared__ T pv[2];
-    reduce_partials<T, 1>(part, nb, sl, ki, pv, KI_BRK + (it & 1));
+    reduce_partials<T, 1>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
     const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
     const bool own = (int)threadIdx.x < KNT / L && i < n;
     const T ri = own ? r[i] : T(0), vi = own ? v[i] : T(0);
@@ -572,7 +578,7 @@ __global__ void __launch_bounds__(KNT) k_abcg_c(int n, int it, const T *__restri
     if (pv[1] != T(0)) return;
     const T rho = ks[KS_RHO + (it & 1)];
     const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
+    if (c.lead && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
     if (own) s[i] = ri - alpha * vi;
 }
 

@@ -235,8 +235,14 @@ class Flow : public FlowBase {
     DevBuf<unsigned> dMomFlags_[D], dVeFlags_;
     DevBuf<double> dMomDelta_[D], dVeDelta_;
     DeviceSystem psys_;
-    DeviceSystem hsys_[D];
+    DeviceSystem hsys_[D];  // per component (HFH_KRYLOV=classic)
     DevBuf<double> hb_[D], hcst_[D];
+    // the D components as one block-diagonal system (fused solves): block d
+    // holds rows [hoff_[d], hoff_[d] + Nf_[d]), blocks padded to 1024 rows
+    DeviceSystem hsysB_;
+    DevBuf<double> hbB_, hxB_;
+    int hoff_[D + 1] = {};
+    bool hbatched_ = false;
     SolveStats stats_, hstats_[D];
     hipEvent_t ev0_[ST_NSTAGE] = {}, ev1_[ST_NSTAGE] = {};
     bool timing_on_ = false;

@@ -184,10 +184,38 @@ void Flow<D>::finalize_layout() {
     }
     psys_.set_matrix(Arp_, Aci_, Av_, st_, true);
     if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
-        for (int d = 0; d < D; d++) {
-            hsys_[d].set_matrix(Hrp_[d], Hci_[d], Hv_[d], st_);
-            hb_[d].alloc(Nf_[d]);
-            hcst_[d].alloc(Nf_[d]);
+        const char *ek = std::getenv("HFH_KRYLOV");
+        hbatched_ = !(ek && std::strcmp(ek, "classic") == 0);
+        for (int d = 0; d < D; d++)
+            if (Nf_[d] == 0) hbatched_ = false;
+        for (int d = 0; d < D; d++) hcst_[d].alloc(Nf_[d]);
+        if (hbatched_) {
+            // block-diagonal system of the D components
+            std::vector<int> rp(1, 0), ci, segs(1, 0);
+            std::vector<double> v;
+            for (int d = 0; d < D; d++) {
+                hoff_[d] = (int)rp.size() - 1;
+                for (int i = 0; i < Nf_[d]; i++) {
+                    for (int k = Hrp_[d][i]; k < Hrp_[d][i + 1]; k++) {
+                        ci.push_back(Hci_[d][k] + hoff_[d]);
+                        v.push_back(Hv_[d][k]);
+                    }
+                    rp.push_back((int)ci.size());
+                }
+                while ((rp.size() - 1) % 1024 != 0) rp.push_back((int)ci.size());
+                segs.push_back((int)rp.size() - 1);
+            }
+            hoff_[D] = (int)rp.size() - 1;
+            hsysB_.set_matrix(rp, ci, v, st_, false, segs);
+            hbB_.alloc(hoff_[D]);
+            hxB_.alloc(hoff_[D]);
+            k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hbB_.p, 0.0);
+            k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hxB_.p, 0.0);
+        } else {
+            for (int d = 0; d < D; d++) {
+                hsys_[d].set_matrix(Hrp_[d], Hci_[d], Hv_[d], st_);
+                hb_[d].alloc(Nf_[d]);
+            }
         }
     }
     refold_constants();
@@ -357,13 +385,29 @@ int Flow<D>::step(bool ve) {
         mp.ve = ve;
         mp.dim = d;
         mp.semi = semi;
-        mp.bout = semi ? hb_[d].p : nullptr;
+        mp.bout = semi ? (hbatched_ ? hbB_.p + hoff_[d] : hb_[d].p) : nullptr;
         mp.bcst = semi ? hcst_[d].p : nullptr;
         if (mp.nf > 0)
             k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[d].p, dMomFlags_[d].p, dMomDelta_[d].p,
                                                              X);
     }
-    if (semi) {  // one Helmholtz solve per component, warm-started from the last u*
+    if (semi && hbatched_) {
+        // the D Helmholtz systems in the same launches, warm-started from
+        // the last u*; one stopping test per component
+        bool spd = true;
+        for (int d = 0; d < D; d++) {
+            spd = spd && Hsym_[d];
+            k_copy<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], X + oUS_[d], hxB_.p + hoff_[d]);
+        }
+        if (par_.precision == 1)
+            hsysB_.fused_mixed(spd, hbB_.p, hxB_.p, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
+        else
+            hsysB_.fused64(spd, hbB_.p, hxB_.p, par_.rtol, par_.atol, par_.maxit, st_);
+        for (int d = 0; d < D; d++) {
+            k_copy<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], hxB_.p + hoff_[d], X + oUS_[d]);
+            if ((int)hsysB_.seg_stats.size() == D) hstats_[d] = hsysB_.seg_stats[d];
+        }
+    } else if (semi) {  // one Helmholtz solve per component, warm-started from the last u*
         for (int d = 0; d < D; d++) {
             if (Nf_[d] == 0) continue;
             double *x = X + oUS_[d];
@@ -438,7 +482,7 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
     mp.ve = par_.flowtype == VISCOELASTIC;
     mp.dim = 0;
     mp.semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
-    mp.bout = mp.semi ? hb_[0].p : nullptr;
+    mp.bout = mp.semi ? (hbatched_ ? hbB_.p : hb_[0].p) : nullptr;
     mp.bcst = mp.semi ? hcst_[0].p : nullptr;
     HFH_CHECK(hipEventRecord(a, st_));
     for (int r = 0; r < reps; r++) {

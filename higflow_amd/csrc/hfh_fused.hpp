@@ -25,14 +25,23 @@ class FusedKrylov {
     bool want_amg = false;
     AmgPrecond<T> amg;
     KBuf<T> ph, sh;
+    // segments (block-diagonal systems): row offsets, multiples of 1024
+    int nseg = 1;
+    std::vector<int> seg_off;
+    KBuf<int> d_wseg, d_wbeg;
+    KSeg S;  // device view; S.wseg == nullptr for one segment
 
     FusedKrylov() = default;
     FusedKrylov(const FusedKrylov &) = delete;
     FusedKrylov &operator=(const FusedKrylov &) = delete;
 
+    // `segs`: row offsets of independent diagonal blocks (first 0, last n,
+    // each a multiple of 1024); empty for one system
     void set(int n_, const int *drp, const int *dci, const std::vector<int> &hrp, const std::vector<int> &hci,
-             const std::vector<double> &hv, hipStream_t st) {
+             const std::vector<double> &hv, hipStream_t st, const std::vector<int> &segs = {}) {
         n = n_;
+        seg_off = segs.empty() ? std::vector<int>{0, n_} : segs;
+        nseg = (int)seg_off.size() - 1;
         rp = drp;
         ci = dci;
         std::vector<double> hd(n, 1.0);
@@ -49,12 +58,29 @@ class FusedKrylov {
         up(vhat, th, st);
         up(dinv, td, st);
         for (KBuf<T> *b : {&r, &rh, &z, &p, &p2, &q, &v, &v2, &s, &t}) b->alloc(n);
-        ks.alloc(KS_N);
-        ki.alloc(KI_N);
+        ks.alloc((size_t)KS_N * nseg);
+        ki.alloc((size_t)KI_N * nseg);
         pred = 0;
+        S = KSeg();
         tune(st);
+        if (nseg > 1) {
+            const int rpw = KNT / lanes;
+            std::vector<int> ws(nb), wb(nseg + 1);
+            for (int q = 0; q <= nseg; q++) wb[q] = seg_off[q] / rpw;
+            for (int q = 0; q < nseg; q++)
+                for (int w = wb[q]; w < wb[q + 1]; w++) ws[w] = q;
+            d_wseg.alloc(nb);
+            d_wbeg.alloc(nseg + 1);
+            if (hipMemcpyAsync(d_wseg.p, ws.data(), nb * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_wbeg.p, wb.data(), (nseg + 1) * sizeof(int), hipMemcpyHostToDevice, st) !=
+                    hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                throw std::runtime_error("segment upload failed");
+            S.wseg = d_wseg.p;
+            S.wbeg = d_wbeg.p;
+        }
         amg.ready = false;
-        if (want_amg && amg.build(n, hrp, hci, hv, st)) {
+        if (want_amg && nseg == 1 && amg.build(n, hrp, hci, hv, st)) {
             ph.alloc(n);
             sh.alloc(n);
         }
@@ -118,7 +144,7 @@ class FusedKrylov {
     template <typename TB, int L>
     void launch_init(bool cg, const TB *b, T *x, hipStream_t st) {
         k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, rp, ci, val.p, x, b, dinv.p, r.p, z.p, rh.p, part.p, nb,
-                                              ki.p);
+                                              ki.p, S);
     }
     template <int L>
     void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
@@ -139,14 +165,15 @@ class FusedKrylov {
             T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
             if (cg) {
                 k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, z.p, pold, pnew, q.p,
-                                                     part.p, nb, ks.p, ki.p);
-                k_kcg_update<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb, ks.p, ki.p);
+                                                     part.p, nb, ks.p, ki.p, S);
+                k_kcg_update<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb, ks.p, ki.p,
+                                                       S);
             } else {
                 T *vold = (it & 1) ? v.p : v2.p, *vnew = (it & 1) ? v2.p : v.p;
                 k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, vhat.p, dinv.p, x, r.p, rh.p, s.p,
-                                                   t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p);
+                                                   t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p, S);
                 k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rp, ci, vhat.p, r.p, rh.p, vnew, s.p, t.p, part.p,
-                                                   nb, ks.p, ki.p);
+                                                   nb, ks.p, ki.p, S);
             }
         }
     }
@@ -158,7 +185,7 @@ class FusedKrylov {
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
             k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, 0, 1 << 30, rp, ci, vhat.p, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
-                                               ks.p, ki.p);
+                                               ks.p, ki.p, KSeg());
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
         float ms = 0;

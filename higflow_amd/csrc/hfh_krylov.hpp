@@ -62,20 +62,50 @@ enum { KI_DONE = 0, KI_ITERS, KI_CONV, KI_BRK, KI_N = KI_BRK + 2 };
 // partial-sum slots (each nb long)
 enum { KP_RZ = 0, KP_RR, KP_BB, KP_PQ, KP_RV, KP_TS, KP_TT, KP_RS, KP_RT, KP_N };
 
+// Block-diagonal systems: rows are split into segments padded to multiples
+// of 1024 rows, each solved as an independent system in the same launches.
+// wseg[w] is the segment of workgroup w, [wbeg[s], wbeg[s+1]) its
+// workgroups; each segment has its own scalars (ks + s KS_N), flags
+// (ki + s KI_N) and partial range.  wseg == nullptr: one segment.
+struct KSeg {
+    const int *wseg = nullptr;
+    const int *wbeg = nullptr;
+};
+struct KCtx {
+    int w0, w1;  // the segment's workgroups
+    bool lead;   // first workgroup of the segment (writes its scalars)
+};
+template <typename T>
+__device__ inline KCtx kctx(const KSeg &S, int nb, T *&ks, int *&ki) {
+    KCtx c;
+    if (S.wseg) {
+        const int sg = S.wseg[blockIdx.x];
+        c.w0 = S.wbeg[sg];
+        c.w1 = S.wbeg[sg + 1];
+        ks += (size_t)sg * KS_N;
+        ki += (size_t)sg * KI_N;
+    } else {
+        c.w0 = 0;
+        c.w1 = nb;
+    }
+    c.lead = (int)blockIdx.x == c.w0;
+    return c;
+}
+
 // Wave 0 sums K partial arrays (nb entries each, fixed order: lane strides,
 // then a butterfly) plus the done flag (slot K) and leaves the totals in
 // `sh`; the other waves go on with their row gathers meanwhile.  Every
 // workgroup computes the same totals.  Read them after kblock_sync().
 template <typename T, int K>
-__device__ inline void reduce_partials(const T *part, int nb, const int (&slot)[K], const int *ki, T *sh,
-                                       int brk = -1) {
+__device__ inline void reduce_partials(const T *part, int nb, const KCtx &c, const int (&slot)[K], const int *ki,
+                                       T *sh, int brk = -1) {
     if (threadIdx.x >= 64) return;
     const int lane = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         T a = 0;
 #pragma unroll 1
-        for (int i = lane; i < nb; i += 64) a += part[(size_t)slot[k] * nb + i];
+        for (int i = c.w0 + lane; i < c.w1; i += 64) a += part[(size_t)slot[k] * nb + i];
         for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
         if (lane == 0) sh[k] = a;
     }
@@ -110,8 +140,8 @@ __device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&sl
 // convergence: a non-finite residual, breakdown, `maxit`, or divergence
 // (||r|| above 1e5 max(||r0||, ||b||), PETSc's default dtol).
 template <typename T>
-__device__ inline bool krylov_stop(bool first, int it, int maxit, T rr, T bb, T rtol, T atol, bool breakdown, T *ks,
-                                   int *ki) {
+__device__ inline bool krylov_stop(bool lead, bool first, int it, int maxit, T rr, T bb, T rtol, T atol, bool breakdown,
+                                   T *ks, int *ki) {
     T tol, rr0;
     if (first) {
         tol = rtol * sqrt(bb);
@@ -125,7 +155,7 @@ __device__ inline bool krylov_stop(bool first, int it, int maxit, T rr, T bb, T 
     const bool conv = finite && sqrt(rr) <= tol;
     const bool diverged = rr > T(1e10) * rr0;
     const bool stop = conv || !finite || breakdown || diverged || it >= maxit;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lead && threadIdx.x == 0) {
         if (first) {
             ks[KS_TOL] = tol;
             ks[KS_BN] = sqrt(bb);
@@ -196,7 +226,9 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, const int *__restr
                                                const T *__restrict__ val, const T *__restrict__ x,
                                                const TB *__restrict__ b, const T *__restrict__ dinv,
                                                T *__restrict__ r, T *__restrict__ z, T *__restrict__ rh,
-                                               T *__restrict__ part, int nb, int *ki) {
+                                               T *__restrict__ part, int nb, int *ki, KSeg S) {
+    T *ksd = nullptr;
+    const KCtx c = kctx<T>(S, nb, ksd, ki);
     const int i = krow<L>();
     const bool in = i < n;
     const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
@@ -221,7 +253,7 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, const int *__restr
     }
     const int sl[3] = {KP_RZ, KP_RR, KP_BB};
     write_partials<T, 3>(acc, part, nb, sl);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (c.lead && threadIdx.x == 0) {
         ki[KI_DONE] = 0;
         ki[KI_ITERS] = 0;
         ki[KI_CONV] = 0;
@@ -238,10 +270,12 @@ __global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rt
                                                   const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const T *__restrict__ val, const T *__restrict__ z,
                                                   const T *__restrict__ pold, T *__restrict__ pnew,
-                                                  T *__restrict__ q, T *__restrict__ part, int nb, T *ks, int *ki) {
+                                                  T *__restrict__ q, T *__restrict__ part, int nb, T *ks, int *ki,
+                                                  KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[3] = {KP_RZ, KP_RR, KP_BB};
     __shared__ T pv[4];
-    reduce_partials<T, 3>(part, nb, sl, ki, pv);
+    reduce_partials<T, 3>(part, nb, c, sl, ki, pv);
     const int i = krow<L>();
     const bool in = i < n;
     const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
@@ -252,13 +286,13 @@ __global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rt
     kblock_sync();
     if (pv[3] != T(0)) return;
     const T rho = pv[0];
-    if (krylov_stop<T>(it == 0, it, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
+    if (krylov_stop<T>(c.lead, it == 0, it, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
     T beta = 0;
     if (it > 0) {
         T rho_old = ks[KS_RHO + ((it + 1) & 1)];
         beta = rho_old != T(0) ? rho / rho_old : T(0);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
+    if (c.lead && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
     T acc[1] = {0};
     if (in && klane<L>() == 0) {
         const T pi = it == 0 ? zi : zi + beta * poi;
@@ -277,10 +311,11 @@ template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_kcg_update(int n, int it, T *__restrict__ x, T *__restrict__ r,
                                                     const T *__restrict__ p, const T *__restrict__ q,
                                                     const T *__restrict__ dinv, T *__restrict__ z,
-                                                    T *__restrict__ part, int nb, const T *ks, const int *ki) {
+                                                    T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[1] = {KP_PQ};
     __shared__ T pv[2];
-    reduce_partials<T, 1>(part, nb, sl, ki, pv);
+    reduce_partials<T, 1>(part, nb, c, sl, ki, pv);
     const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
     const bool own = (int)threadIdx.x < KNT / L && i < n;
     T xi = 0, ri = 0, pi = 0, qi = 0, di = 0;
@@ -323,7 +358,8 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
                                                 const T *__restrict__ s, const T *__restrict__ t,
                                                 const T *__restrict__ pold, const T *__restrict__ vold,
                                                 T *__restrict__ pnew, T *__restrict__ vnew, T *__restrict__ part,
-                                                int nb, T *ks, int *ki) {
+                                                int nb, T *ks, int *ki, KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int i = krow<L>();
     const bool in = i < n;
     const bool own = in && klane<L>() == 0;
@@ -332,15 +368,15 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
     if (it == 0) {
         const int sl[3] = {KP_RZ, KP_RR, KP_BB};
         __shared__ T pv[4];
-        reduce_partials<T, 3>(part, nb, sl, ki, pv);
+        reduce_partials<T, 3>(part, nb, c, sl, ki, pv);
         const T *const vr[1] = {r};
         T sr[1];
         row_sums<T, L, 1>(rb, re, ci, vhat, vr, sr);
         const T ri = in ? r[i] : T(0), rhi = in ? rh[i] : T(0);
         kblock_sync();
         if (pv[3] != T(0)) return;
-        if (krylov_stop<T>(true, 0, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
-        if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO] = pv[0];
+        if (krylov_stop<T>(c.lead, true, 0, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
+        if (c.lead && threadIdx.x == 0) ks[KS_RHO] = pv[0];
         if (own) {
             pnew[i] = ri;
             vnew[i] = sr[0];
@@ -350,7 +386,7 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
     } else {
         const int sl[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
         __shared__ T pv[5];
-        reduce_partials<T, 4>(part, nb, sl, ki, pv, KI_BRK + ((it + 1) & 1));
+        reduce_partials<T, 4>(part, nb, c, sl, ki, pv, KI_BRK + ((it + 1) & 1));
         const T *const vv[4] = {s, t, pold, vold};
         T sv[4];
         row_sums<T, L, 4>(rb, re, ci, vhat, vv, sv);
@@ -380,14 +416,14 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
         }
         if (breakdown) {
             // x holds this iteration's update; the host checks the true residual
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (c.lead && threadIdx.x == 0) {
                 ki[KI_ITERS] = it;
                 ki[KI_CONV] = 0;
                 ki[KI_BRK + (it & 1)] = 1;
             }
             return;
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
+        if (c.lead && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
         if (own) {
             const T vi = (sv[0] - omega * sv[1]) + beta * (sv[2] - omega * sv[3]);
             pnew[i] = ri + beta * (poi - omega * voi);
@@ -409,10 +445,11 @@ __global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, const 
                                                 const int *__restrict__ ci, const T *__restrict__ vhat,
                                                 const T *__restrict__ r, const T *__restrict__ rh,
                                                 const T *__restrict__ v, T *__restrict__ s, T *__restrict__ t,
-                                                T *__restrict__ part, int nb, T *ks, int *ki) {
+                                                T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[2] = {KP_RV, KP_RR};
     __shared__ T pv[3];
-    reduce_partials<T, 2>(part, nb, sl, ki, pv, KI_BRK + (it & 1));
+    reduce_partials<T, 2>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
     const int i = krow<L>();
     const bool in = i < n;
     const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
@@ -422,10 +459,10 @@ __global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, const 
     const T ri = in ? r[i] : T(0), vi = in ? v[i] : T(0), rhi = in ? rh[i] : T(0);
     kblock_sync();
     if (pv[2] != T(0)) return;
-    if (it > 0 && krylov_stop<T>(false, it, maxit, pv[1], T(0), T(0), T(0), false, ks, ki)) return;
+    if (it > 0 && krylov_stop<T>(c.lead, false, it, maxit, pv[1], T(0), T(0), T(0), false, ks, ki)) return;
     const T rho = ks[KS_RHO + (it & 1)];
     const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
+    if (c.lead && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
     T acc[4] = {0, 0, 0, 0};
     if (in && klane<L>() == 0) {
         const T si = ri - alpha * vi;

@@ -382,6 +382,13 @@ class DeviceSystem {
     int max_restarts = 1;      // fused64: restarts on the true residual
     double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
     int since_probe_ = 0;
+    int nseg = 1;
+    std::vector<int> seg_off;
+    std::vector<SolveStats> seg_stats;  // last solve, per segment
+    std::vector<int> seg_hi_;
+    std::vector<double> seg_hs_;
+    int off_ks_ = 64, off_sc_ = 256;
+    size_t hpin_size_ = 0;
     bool force32_ = false;  // HFH_MIXED_FP32=1: always refine with FP32 solves
     DevBuf<double> xbak;
 
@@ -394,8 +401,10 @@ class DeviceSystem {
 
     // `amg`: precondition the fused BiCGStab with a smoothed-aggregation
     // V-cycle (HFH_AMG=0 turns it off)
+    // `segs`: row offsets of independent diagonal blocks (multiples of 1024;
+    // see set_blocks), solved together with one stopping test each
     void set_matrix(const std::vector<int> &hrp, const std::vector<int> &hci, const std::vector<double> &hv,
-                    hipStream_t st, bool amg = false) {
+                    hipStream_t st, bool amg = false, const std::vector<int> &segs = {}) {
         n = (int)hrp.size() - 1;
         nnz = (int)hci.size();
         rp.upload(hrp.data(), hrp.size(), st);
@@ -421,7 +430,20 @@ class DeviceSystem {
             if ((int)b->n < nb) b->alloc(nb);
         if (!sc.p) sc.alloc(S_NSC);
         if (!fsc.p) fsc.alloc(S_NSC);
-        if (!hpin_) HFH_CHECK(hipHostMalloc((void **)&hpin_, 512, hipHostMallocDefault));
+        nseg = segs.empty() ? 1 : (int)segs.size() - 1;
+        seg_off = segs.empty() ? std::vector<int>{0, n} : segs;
+        off_ks_ = ((nseg * KI_N * (int)sizeof(int) + 63) / 64) * 64;
+        off_sc_ = off_ks_ + ((nseg * KS_N * (int)sizeof(double) + 63) / 64) * 64;
+        const size_t pin = (size_t)off_sc_ + 256;
+        if (hpin_ && pin > hpin_size_) {
+            (void)hipHostFree(hpin_);
+            hpin_ = nullptr;
+        }
+        if (!hpin_) {
+            HFH_CHECK(hipHostMalloc((void **)&hpin_, pin, hipHostMallocDefault));
+            hpin_size_ = pin;
+        }
+        seg_stats.assign(nseg, SolveStats());
         if ((int)xbak.n < n) xbak.alloc(n);
         if (const char *er = std::getenv("HFH_RESTARTS")) max_restarts = std::max(0, std::atoi(er));
         {
@@ -434,8 +456,8 @@ class DeviceSystem {
             const char *em = std::getenv("HFH_MIXED_FP32");
             force32_ = em && std::strcmp(em, "1") == 0;
         }
-        k64.set(n, rp.p, ci.p, hrp, hci, hv, st);
-        k32.set(n, rp.p, ci.p, hrp, hci, hv, st);
+        k64.set(n, rp.p, ci.p, hrp, hci, hv, st, segs);
+        k32.set(n, rp.p, ci.p, hrp, hci, hv, st, segs);
         // HFH_KRYLOV=classic selects the one-kernel-per-operation iterations
         const char *e = std::getenv("HFH_KRYLOV");
         fused = !(e && std::strcmp(e, "classic") == 0);
@@ -443,16 +465,38 @@ class DeviceSystem {
     }
 
     // ------------------------------------------------- fused iterations
-    // copies the flags and scalars of one fused solve to pinned host memory
+    // Copies the flags and scalars of one fused solve (all segments) to
+    // pinned host memory; per-segment values go to seg_hi_ / seg_hs_, hi/hs
+    // get the aggregate: done and converged when every segment is, the
+    // largest iteration count, the scalars of the segment with the largest
+    // relative residual.
     template <typename T>
     void read_status(const FusedKrylov<T> &k, int *hi, double *hs, hipStream_t st) {
-        HFH_CHECK(hipMemcpyAsync(hpin_, k.ki.p, KI_N * sizeof(int), hipMemcpyDeviceToHost, st));
-        HFH_CHECK(hipMemcpyAsync(hpin_ + 64, k.ks.p, KS_N * sizeof(T), hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipMemcpyAsync(hpin_, k.ki.p, (size_t)nseg * KI_N * sizeof(int), hipMemcpyDeviceToHost, st));
+        HFH_CHECK(hipMemcpyAsync(hpin_ + off_ks_, k.ks.p, (size_t)nseg * KS_N * sizeof(T), hipMemcpyDeviceToHost, st));
         HFH_CHECK(hipStreamSynchronize(st));
-        std::memcpy(hi, hpin_, KI_N * sizeof(int));
-        if (hi[KI_BRK] || hi[KI_BRK + 1]) hi[KI_DONE] = 1;  // breakdown: stopped, not converged
-        const T *t = reinterpret_cast<const T *>(hpin_ + 64);
-        for (int q = 0; q < KS_N; q++) hs[q] = (double)t[q];
+        seg_hi_.resize((size_t)nseg * KI_N);
+        seg_hs_.resize((size_t)nseg * KS_N);
+        std::memcpy(seg_hi_.data(), hpin_, (size_t)nseg * KI_N * sizeof(int));
+        const T *t = reinterpret_cast<const T *>(hpin_ + off_ks_);
+        for (int q = 0; q < nseg * KS_N; q++) seg_hs_[q] = (double)t[q];
+        hi[KI_DONE] = 1;
+        hi[KI_CONV] = 1;
+        hi[KI_ITERS] = 0;
+        double worst = -1;
+        for (int g = 0; g < nseg; g++) {
+            int *gi = &seg_hi_[(size_t)g * KI_N];
+            if (gi[KI_BRK] || gi[KI_BRK + 1]) gi[KI_DONE] = 1;  // breakdown: stopped, not converged
+            hi[KI_DONE] &= gi[KI_DONE] ? 1 : 0;
+            hi[KI_CONV] &= gi[KI_CONV] ? 1 : 0;
+            hi[KI_ITERS] = std::max(hi[KI_ITERS], gi[KI_ITERS]);
+            const double *gs = &seg_hs_[(size_t)g * KS_N];
+            const double rel = gs[KS_BN] > 0 ? std::sqrt(gs[KS_RR]) / gs[KS_BN] : std::sqrt(gs[KS_RR]);
+            if (rel > worst || g == 0) {
+                worst = rel;
+                for (int q = 0; q < KS_N; q++) hs[q] = gs[q];
+            }
+        }
     }
 
     // runs one fused solve to its stopping test: the predicted number of
@@ -479,46 +523,67 @@ class DeviceSystem {
     // restarted from x (at most twice) while it misses the tolerance.
     SolveStats fused64(bool spd, const double *b, double *x, double rtol, double atol, int maxit, hipStream_t st) {
         int hi[KI_N];
-        double hs[KS_N], h[S_NSC];
+        double hs[KS_N];
         SolveStats S;
-        double best = -1.0;  // true residual norm of the kept x
+        double best = -1.0;  // largest true relative residual over segments of the kept x
+        std::vector<double> rr(nb), bb(nb);
         for (int restart = 0; restart <= max_restarts; restart++) {
             if (restart > 0)  // keep the current x in case the restart does worse
                 HFH_CHECK(hipMemcpyAsync(xbak.p, x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
             k64.init<double>(spd, b, x, st);
             S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
+            // true residual per segment (segments start on 256-row blocks)
             k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
-            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
-            HFH_CHECK(hipMemcpyAsync(hpin_ + 256, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipMemcpyAsync(rr.data(), part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipMemcpyAsync(bb.data(), part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipStreamSynchronize(st));
-            std::memcpy(h, hpin_ + 256, sizeof h);
-            const double bn = std::sqrt(h[S_BB]), rn = std::sqrt(h[S_RR]);
-            const double tol = std::max(rtol * bn, atol);
-            if (restart > 0 && !(rn < best)) {  // no better: restore
+            bool all_conv = true, worth = false;
+            double worst = 0;
+            std::vector<SolveStats> gs(nseg);
+            for (int g = 0; g < nseg; g++) {
+                double sr = 0, sb = 0;
+                for (int w = seg_off[g] / BLOCK; w < std::min(nb, (seg_off[g + 1] + BLOCK - 1) / BLOCK); w++) {
+                    sr += rr[w];
+                    sb += bb[w];
+                }
+                const double bn = std::sqrt(sb), rn = std::sqrt(sr), tol = std::max(rtol * bn, atol);
+                const int *gi = &seg_hi_[(size_t)g * KI_N];
+                gs[g].iters = gi[KI_DONE] ? gi[KI_ITERS] : S.iters;
+                gs[g].resid = bn > 0 ? rn / bn : rn;
+                gs[g].converged = rn <= tol;
+                all_conv = all_conv && gs[g].converged;
+                worst = std::max(worst, gs[g].resid);
+                // restart when the recursion claims convergence the true
+                // residual misses by more than a factor 10
+                if (!gs[g].converged && gi[KI_DONE] && gi[KI_CONV] && rn > 10.0 * tol) worth = true;
+            }
+            if (restart > 0 && !(worst < best)) {  // no better: restore
                 HFH_CHECK(hipMemcpyAsync(x, xbak.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
                 break;
             }
-            best = rn;
-            S.resid = bn > 0 ? rn / bn : rn;
-            S.converged = rn <= tol;
-            // restart only when the recursion claims convergence the true
-            // residual misses by more than a factor 10
-            if (S.converged || !(hi[KI_DONE] && hi[KI_CONV]) || rn <= 10.0 * tol || S.iters >= maxit) break;
+            best = worst;
+            seg_stats = gs;
+            S.resid = worst;
+            S.converged = all_conv;
+            if (all_conv || !worth || S.iters >= maxit) break;
             S.restarts++;
         }
+        for (auto &g : seg_stats) g.restarts = S.restarts;
         return S;
     }
 
-    // Mixed: FP64 defect correction around fused FP32 solves (PCG or
-    // BiCGStab) of A e = r from e = 0 to inner_rtol.  Each inner solve runs
-    // to its own tolerance (a truncated BiCGStab correction can raise the
-    // error).
     // Mixed form: per system, FP32 refinement or the FP64 solve, whichever
     // took less wall time on its last solves (both are timed on the host
     // clock; each solve ends with a host synchronisation).  The form not in
     // use is re-timed every 50 solves.
     SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                            int maxit, hipStream_t st) {
+        if (nseg > 1) {  // several blocks: the FP64 solve
+            SolveStats S = fused64(spd, b, x, rtol, atol, maxit, st);
+            S.fallback = 2;
+            for (auto &g : seg_stats) g.fallback = 2;
+            return S;
+        }
         bool use32;
         if (force32_)
             use32 = true;
@@ -570,9 +635,9 @@ class DeviceSystem {
         for (;;) {
             k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
             k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
-            HFH_CHECK(hipMemcpyAsync(hpin_ + 256, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipMemcpyAsync(hpin_ + off_sc_, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipStreamSynchronize(st));
-            std::memcpy(h, hpin_ + 256, sizeof h);
+            std::memcpy(h, hpin_ + off_sc_, sizeof h);
             bn = std::sqrt(h[S_BB]);
             rn = std::sqrt(h[S_RR]);
             tol = std::max(rtol * bn, atol);
