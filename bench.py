@@ -35,6 +35,10 @@ WORKLOADS = {
     # identity, semi-implicit Euler, 200 steps.  dt=2e-5: at dt=1e-4 the
     # restated algorithm itself diverges (the CPU oracle at step 17).
     "contraction2d_3lev_oldroydb": dict(kind="contraction", levels=3, dt=2e-5, steps=200),
+    # the same contraction on a finer, graded mesh: base cells 0.05 and 4
+    # levels whose boxes shrink with the level (radius 1.0, 0.75, 0.5,
+    # 0.25), so each level boundary is a 2:1 jump (121,000 leaves), dt=1e-5
+    "contraction2d_4lev_graded": dict(kind="contraction", levels=4, graded=True, h0=0.05, dt=1e-5, steps=200),
     # 2D Newtonian lid-driven cavity, uniform 64x64, Re=100, dt=1e-3,
     # explicit Euler, incremental projection, CUBISTA, 100 steps
     "cavity2d_64_re100": dict(kind="cavity", n=64, Re=100.0, dt=1e-3, steps=100),
@@ -51,7 +55,8 @@ def make_case(spec, **tol):
     if spec["kind"] == "cavity":
         return cases.cavity(spec["n"], Re=spec["Re"], dt=spec["dt"], **tol)
     if spec["kind"] == "contraction":
-        return cases.contraction(levels=spec["levels"], dt=spec["dt"], **tol)
+        return cases.contraction(levels=spec["levels"], dt=spec["dt"], graded=spec.get("graded", False),
+                                 h0=spec.get("h0", 0.1), **tol)
     return cases.ve_cavity(spec["n"], Re=spec["Re"], Wi=spec["Wi"], beta=spec["beta"], dt=spec["dt"], **tol)
 
 

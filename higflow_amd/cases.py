@@ -137,8 +137,9 @@ def newt_channel(nx=160, ny=40, **kw):
 def _contraction_amr(lo, hi, h0, levels, boxes):
     """.amr text of one tree: base spacing h0 on [lo, hi], then `levels`
     factor-2 refinement levels over the cells inside any of `boxes`
-    ((xlo, xhi, ylo, yhi) in absolute coordinates).  Patches follow the
-    reference's convention: level delta, 1-based first cell, cell counts."""
+    ((xlo, xhi, ylo, yhi) in absolute coordinates; a callable lv -> boxes
+    gives each level its own boxes).  Patches follow the reference's
+    convention: level delta, 1-based first cell, cell counts."""
     nx = int(round((hi[0] - lo[0]) / h0))
     ny = int(round((hi[1] - lo[1]) / h0))
     lines = ["%s %s %s %s" % (_fmt(lo[0]), _fmt(hi[0]), _fmt(lo[1]), _fmt(hi[1])), str(1 + levels),
@@ -146,7 +147,7 @@ def _contraction_amr(lo, hi, h0, levels, boxes):
     for lv in range(1, levels + 1):
         h = h0 / (2 ** lv)
         patches = []
-        for (bx0, bx1, by0, by1) in boxes:
+        for (bx0, bx1, by0, by1) in (boxes(lv) if callable(boxes) else boxes):
             x0, x1 = max(bx0, lo[0]), min(bx1, hi[0])
             y0, y1 = max(by0, lo[1]), min(by1, hi[1])
             if x1 <= x0 or y1 <= y0:
@@ -162,7 +163,7 @@ def _contraction_amr(lo, hi, h0, levels, boxes):
 
 
 def contraction(levels=3, h0=0.1, length=20.0, H=4.0, h=1.0, radius=1.0, Re=0.01, Wi=1.0,
-                beta=1.0 / 9.0, dt=1e-4, viscoelastic=True, **kw):
+                beta=1.0 / 9.0, dt=1e-4, viscoelastic=True, graded=False, **kw):
     """Planar 4:1 contraction: upstream channel [-L/2, 0] x [-H, H], downstream
     [0, L/2] x [-h, h] as two HiG-Tree trees (the layout of
     higflow/example2d_Newt_contraction), base spacing h0 and `levels` levels
@@ -170,7 +171,17 @@ def contraction(levels=3, h0=0.1, length=20.0, H=4.0, h=1.0, radius=1.0, Re=0.01
     (0, +-h).  Parabolic inflow with mean 1, no-slip walls, outflow with
     velocity Neumann / pressure Dirichlet.  Oldroyd-B, log-conformation."""
     Lh = length / 2.0
-    boxes = [(-radius, radius, h - radius, h + radius), (-radius, radius, -h - radius, -h + radius)]
+
+    def corner_boxes(r):
+        # boxes snapped outward to the base grid so every level stays aligned
+        r = math.ceil(r / h0 - 1e-9) * h0
+        return [(-r, r, h - r, h + r), (-r, r, -h - r, -h + r)]
+    if graded:
+        # level lv refines within radius (levels - lv + 1) / levels: each
+        # level boundary is a 2:1 jump instead of one 2^levels:1 jump
+        boxes = lambda lv: corner_boxes(radius * (levels - lv + 1) / levels)
+    else:
+        boxes = corner_boxes(radius)
     up = _contraction_amr((-Lh, -H), (0.0, H), h0, levels, boxes)
     dn = _contraction_amr((0.0, -h), (Lh, h), h0, levels, boxes)
     nyu, nyd, nx = int(round(2 * H / h0)), int(round(2 * h / h0)), int(round(Lh / h0))
@@ -204,4 +215,4 @@ def contraction(levels=3, h0=0.1, length=20.0, H=4.0, h=1.0, radius=1.0, Re=0.01
         kw.setdefault("kernel_transform", 1)
     params = default_params(Re=Re, dt=dt, De=Wi, beta=beta, **kw)
     return dict(dim=2, domains=[up, dn], bcs=bcs, params=params,
-                name="contraction_%dlev_h%s" % (levels, f(h0)), ncells=None)
+                name="contraction_%dlev%s_h%s" % (levels, "_graded" if graded else "", f(h0)), ncells=None)

@@ -329,6 +329,22 @@ __global__ void __launch_bounds__(1024) k_amg_coarse(int nc, const int *__restri
     }
 }
 
+// coarsest level, many workgroups: x_c = C^-1 b_c with b_c already
+// restricted (a wave per row)
+template <typename T>
+__global__ void __launch_bounds__(ANT) k_amg_dense(int nc, const T *__restrict__ cinv, const T *__restrict__ bc,
+                                                   T *__restrict__ xc, const int *ki) {
+    if (amg_done(ki)) return;
+    const int lane = threadIdx.x & 63;
+    const int i = (blockIdx.x * ANT + threadIdx.x) >> 6;
+    if (i >= nc) return;
+    const T *row = cinv + (size_t)i * nc;
+    T s = 0;
+    for (int j = lane; j < nc; j += 64) s += row[j] * bc[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) xc[i] = s;
+}
+
 // x = omega D^-1 b + P x_c
 template <typename T>
 __global__ void __launch_bounds__(ANT) k_amg_prolong(int n, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -374,7 +390,10 @@ class AmgPrecond {
         std::vector<AmgHostLevel> hl;
         std::vector<double> cinv;
         ready = false;
-        if (!sa_build(A, hl, cinv, ncoarse)) return false;
+        // stop coarsening at <= 1000 unknowns: one dense solve of up to
+        // 1000 rows (two launches) is cheaper than further levels (four
+        // launches each)
+        if (!sa_build(A, hl, cinv, ncoarse, 1000)) return false;
         nlev = (int)hl.size();
         lv_.clear();
         lv_.resize(nlev);
@@ -442,9 +461,14 @@ class AmgPrecond {
         if (l + 1 < nlev) {
             k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
             cycle(l + 1, d.bnext.p, d.xnext.p, ki, st);
-        } else {
+        } else if (ncoarse <= 256) {
             k_amg_coarse<T><<<1, 1024, ncoarse * sizeof(T), st>>>(ncoarse, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, cinv_.p,
                                                                    d.xnext.p, ki);
+        } else {
+            // larger coarsest level: restrict, then a wave per dense row
+            k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
+            k_amg_dense<T><<<std::max(1, (ncoarse * 64 + ANT - 1) / ANT), ANT, 0, st>>>(ncoarse, cinv_.p, d.bnext.p,
+                                                                                        d.xnext.p, ki);
         }
         k_amg_prolong<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b, d.xnext.p,
                                                     d.x.p, ki);

@@ -602,6 +602,15 @@ class DeviceSystem {
         SolveStats S;
         if (use32) {
             S = mixed_refine(spd, b, x, rtol, atol, inner_rtol, maxit, st);
+            if (!S.converged && S.fallback == 0) {
+                // the refinement stalled (FP32 too coarse for this system):
+                // finish in FP64 from the current x, keep FP64 from now on
+                SolveStats F = fused64(spd, b, x, rtol, atol, maxit, st);
+                F.iters += S.iters;
+                F.outer = S.outer;
+                F.fallback = 1;
+                S = F;
+            }
         } else {
             S = fused64(spd, b, x, rtol, atol, maxit, st);
             S.fallback = 2;

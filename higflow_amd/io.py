@@ -1,0 +1,166 @@
+"""Reference file formats around the step.
+
+* ``load_reference_inputs(directory, prefix, bc_value=None)`` builds a case
+  from a HiG-Flow example's input files:
+  * ``<prefix>.domain.yaml`` and ``<prefix>.bc.yaml``, with the ``.amr``
+    trees they name;
+  * ``<prefix>.par.contr.yaml``;
+  * ``<prefix>.init.yaml``.
+
+  These are what higflow_initialize_* reads (higflow/src/hig-flow-io.c,
+  hig-flow-bc.c:754-866). Boundary values are the driver's C callbacks in the
+  reference (ns-example-2d.c:46-100), so they are passed in as
+  ``bc_value(bc_id, kind, x)``; the default is zero.
+* ``save_properties(sim, prefix)`` and ``load_properties(sim, prefix)`` write
+  and read the checkpoint files of higflow_save_properties /
+  higflow_load_properties (hig-flow-io.c:5716-5990, pdomain.c:1328-1600):
+  * one file per component: ``<prefix>._dpp``, ``._dpu<d>``, ``._dpF``,
+    ``._dpFU<d>`` and, viscoelastic, ``._dpKernel<i><j>``;
+  * FP64 records of the point (DIM coordinates) followed by the value, in
+    the local-id order of one rank.
+
+  Loading locates each record by its point, as the reference does, so the
+  record order does not matter.
+
+``sim`` is anything with ``size/centers/get/set`` of the field ids of
+``higflow_amd.native.Flow`` (the oracle binding has the same interface).
+"""
+import os
+
+import numpy as np
+import yaml
+
+from . import cases
+
+# field ids: 0 u[d], 1 p, 4 F, 5 FU[d], 6 kernel[i*D+j]
+_SCALARS = (("_dpp", 1), ("_dpF", 4))
+_VECTORS = (("_dpu", 0), ("_dpFU", 5))
+
+_PROJ = {"incremental": cases.INCREMENTAL, "non_incremental": cases.NON_INCREMENTAL}
+_TEMP = {"explicit_euler": cases.EXPLICIT_EULER, "semi_implicit_euler": cases.SEMI_IMPLICIT_EULER}
+_CONV = {"central": cases.CENTRAL, "first_order": cases.FIRST_ORDER, "second_order": cases.SECOND_ORDER}
+_SECOND = {"modified_coefficient_upwind": cases.MCU, "cubista": cases.CUBISTA, "quick": cases.QUICK}
+_FLOW = {"newtonian": cases.NEWTONIAN, "viscoelastic": cases.VISCOELASTIC}
+_VEMODEL = {"oldroyd_b": cases.OLDROYD_B, "giesekus": cases.GIESEKUS}
+_VECONV = {"upwind": cases.CELL_CENTRAL, "central": cases.CELL_CENTRAL, "cubista": cases.CELL_CUBISTA}
+_BCTYPE = {"dirichlet": cases.DIRICHLET, "neumann": cases.NEUMANN}
+
+
+def _yaml(path):
+    with open(path) as f:
+        return yaml.safe_load(f)
+
+
+def _lookup(table, key, what):
+    if key not in table:
+        raise ValueError("%s '%s' is not supported by the GPU step (supported: %s)"
+                         % (what, key, ", ".join(sorted(table))))
+    return table[key]
+
+
+def _amr_dim(text):
+    return len(text.split("\n", 1)[0].split()) // 2
+
+
+def load_reference_inputs(directory, prefix, bc_value=None, **param_over):
+    """A case dict from a HiG-Flow example's input files.
+
+    Supported: single-phase Newtonian, or viscoelastic Oldroyd-B/Giesekus
+    with the log-conformation kernel. Boundary values come from
+    ``bc_value(bc_id, kind, x)`` (kind 0 pressure, 1+d velocity d). Other
+    options raise ValueError.
+    """
+    p = lambda name: os.path.join(directory, "%s.%s.yaml" % (prefix, name))
+    dom = _yaml(p("domain"))["domain"]
+    domains = []
+    for k in range(int(dom["number_domains"])):
+        with open(os.path.join(directory, dom["domain%d" % k]["path"])) as f:
+            domains.append(f.read())
+    dim = _amr_dim(domains[0])
+    bcy = _yaml(p("bc"))["bc"]
+    bcs = []
+    for k in range(int(bcy["number_bc"])):
+        e = bcy["bc%d" % k]
+        with open(os.path.join(directory, e["path"])) as f:
+            amr = f.read()
+        bid = int(e["id"])
+        ptype = _lookup(_BCTYPE, e["pressure"]["type"], "pressure boundary type")
+        utypes = [_lookup(_BCTYPE, e["velocity_%d" % d]["type"], "velocity boundary type") for d in range(dim)]
+        if bc_value is None:
+            val = (lambda kind, x: 0.0)
+        else:
+            val = (lambda b: (lambda kind, x: bc_value(b, kind, x)))(bid)
+        bcs.append(dict(id=bid, amr=amr, ptype=ptype, utypes=utypes, value=val))
+    par = _yaml(p("par.contr"))
+    sp, contr = par["simulation_par"], par["simulation_contr"]
+    if par.get("flowphase", "singlephase") != "singlephase":
+        raise ValueError("flowphase '%s' is not supported by the GPU step" % par["flowphase"])
+    single = par["singlephase"]
+    flowtype = _lookup(_FLOW, single["contr"]["flowtype"], "flowtype")
+    kw = dict(Re=float(par["adimensional"]["Re"]), dt=float(sp["dt"]),
+              projtype=_lookup(_PROJ, contr["projtype"], "projtype"),
+              tempdiscrtype=_lookup(_TEMP, contr["tempdiscrtype"], "tempdiscrtype"),
+              convecdiscrtype=_lookup(_CONV, contr["convecdiscrtype"], "convecdiscrtype"),
+              secondconvecdiscrtype=_lookup(_SECOND, contr["secondconvecdiscrtype"], "secondconvecdiscrtype"),
+              flowtype=flowtype)
+    if flowtype == cases.VISCOELASTIC:
+        ve = single["viscoelastic"]
+        kw.update(ve_model=_lookup(_VEMODEL, ve["contr"]["model"], "viscoelastic model"),
+                  ve_convec=_lookup(_VECONV, ve["contr"]["convecdiscrtype"], "viscoelastic convecdiscrtype"),
+                  De=float(ve["adimensional"]["De"]), beta=float(ve["adimensional"]["beta"]),
+                  kernel_tol=float(ve["adimensional"]["kernel_tol"]),
+                  giesekus_alpha=float(ve.get("model_giesekus", {}).get("alpha", 0.2)))
+    if os.path.exists(p("init")):
+        kw["t"] = float(_yaml(p("init"))["init_par"]["t"])
+    kw.update(param_over)
+    return dict(dim=dim, domains=domains, bcs=bcs, params=cases.default_params(**kw),
+                name="%s/%s" % (os.path.basename(os.path.normpath(directory)), prefix), ncells=None,
+                numsteps=int(sp.get("numsteps", 0)))
+
+
+def _components(sim, dim, ve):
+    out = [(suf, f, 0) for suf, f in _SCALARS]
+    out += [("%s%d" % (suf, d), f, d) for suf, f in _VECTORS for d in range(dim)]
+    if ve:
+        out += [("_dpKernel%d%d" % (i, j), 6, i * dim + j) for i in range(dim) for j in range(dim)]
+    return out
+
+
+def save_properties(sim, prefix, dim=2, viscoelastic=False):
+    """Writes the fields in the reference's checkpoint format."""
+    for suf, field, comp in _components(sim, dim, viscoelastic):
+        c = np.asarray(sim.centers(field, comp), dtype=np.float64).reshape(-1, dim)
+        v = np.asarray(sim.get(field, comp), dtype=np.float64)
+        rec = np.empty((len(v), dim + 1))
+        rec[:, :dim] = c
+        rec[:, dim] = v
+        rec.tofile(prefix + "." + suf)
+
+
+def _key(c):
+    return tuple(np.round(np.asarray(c) * 1e9).astype(np.int64))
+
+
+def load_properties(sim, prefix, dim=2, viscoelastic=False, missing="error"):
+    """Reads checkpoint files written by the reference (or save_properties)
+    into ``sim``, matching each record to the element with that centre.
+    ``missing='skip'`` leaves components whose file is absent untouched."""
+    for suf, field, comp in _components(sim, dim, viscoelastic):
+        path = prefix + "." + suf
+        if not os.path.exists(path):
+            if missing == "skip":
+                continue
+            raise FileNotFoundError(path)
+        rec = np.fromfile(path, dtype=np.float64).reshape(-1, dim + 1)
+        c = np.asarray(sim.centers(field, comp), dtype=np.float64).reshape(-1, dim)
+        where = {_key(c[i]): i for i in range(len(c))}
+        vals = np.asarray(sim.get(field, comp), dtype=np.float64).copy()
+        hit = 0
+        for r in rec:
+            i = where.get(_key(r[:dim]))
+            if i is not None:
+                vals[i] = r[dim]
+                hit += 1
+        if hit != len(c):
+            raise ValueError("%s: %d of %d elements found in the file" % (path, hit, len(c)))
+        sim.set(field, comp, vals)

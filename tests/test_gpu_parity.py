@@ -189,3 +189,15 @@ def test_mixed_precision_amr_contraction(force32, monkeypatch):
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
     a, b = hmx.get(1), h64.get(1)
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+
+
+def test_reference_example_inputs():
+    # the reference example's own input files (higflow/example2d_Newt/input),
+    # loaded as a case, through the HIP step against the oracle
+    import os
+    from higflow_amd import io
+    ex = os.path.join(os.path.dirname(__file__), "golden", "example2d_newt")
+
+    def inflow(bid, kind, x):
+        return 1.5 * (1.0 - x[1] * x[1]) if (bid == 0 and kind == 1) else 0.0
+    compare(io.load_reference_inputs(ex, "input/example-2d.load", bc_value=inflow, **TIGHT), 3)
