@@ -134,32 +134,66 @@ def newt_channel(nx=160, ny=40, **kw):
                 ncells=nx * ny)
 
 
-def _contraction_amr(lo, hi, h0, levels, boxes):
-    """.amr text of one tree: base spacing h0 on [lo, hi], then `levels`
-    factor-2 refinement levels over the cells inside any of `boxes`
-    ((xlo, xhi, ylo, yhi) in absolute coordinates; a callable lv -> boxes
-    gives each level its own boxes).  Patches follow the reference's
-    convention: level delta, 1-based first cell, cell counts."""
-    nx = int(round((hi[0] - lo[0]) / h0))
-    ny = int(round((hi[1] - lo[1]) / h0))
-    lines = ["%s %s %s %s" % (_fmt(lo[0]), _fmt(hi[0]), _fmt(lo[1]), _fmt(hi[1])), str(1 + levels),
-             "%s %s 1" % (_fmt(h0), _fmt(h0)), "1 1 %d %d" % (nx, ny)]
+def amr_refined(lo, hi, h0, levels, boxes):
+    """.amr text of one tree in any dimension: base spacing h0 on [lo, hi],
+    then `levels` factor-2 refinement levels over the cells inside any of
+    `boxes` ((blo, bhi) corner pairs in absolute coordinates; a callable
+    lv -> boxes gives each level its own boxes).  Patches follow the
+    reference's convention (higtree/src/higtree-io.c:97-153, 269-293): level
+    delta and patch count, then per patch the 1-based first cell (odd: the
+    point delta*initcell is the centre of a parent cell) and the cell
+    counts.  Boxes must be aligned to the parent level's grid."""
+    dim = len(lo)
+    n = [int(round((hi[d] - lo[d]) / h0)) for d in range(dim)]
+    lines = [" ".join("%s %s" % (_fmt(lo[d]), _fmt(hi[d])) for d in range(dim)), str(1 + levels),
+             " ".join([_fmt(h0)] * dim) + " 1", " ".join(["1"] * dim + [str(v) for v in n])]
     for lv in range(1, levels + 1):
         h = h0 / (2 ** lv)
         patches = []
-        for (bx0, bx1, by0, by1) in (boxes(lv) if callable(boxes) else boxes):
-            x0, x1 = max(bx0, lo[0]), min(bx1, hi[0])
-            y0, y1 = max(by0, lo[1]), min(by1, hi[1])
-            if x1 <= x0 or y1 <= y0:
+        for (blo, bhi) in (boxes(lv) if callable(boxes) else boxes):
+            a = [max(blo[d], lo[d]) for d in range(dim)]
+            b = [min(bhi[d], hi[d]) for d in range(dim)]
+            if any(b[d] <= a[d] for d in range(dim)):
                 continue
-            i0 = int(round((x0 - lo[0]) / h))
-            j0 = int(round((y0 - lo[1]) / h))
-            ni = int(round((x1 - x0) / h))
-            nj = int(round((y1 - y0) / h))
-            patches.append("%d %d %d %d" % (i0 + 1, j0 + 1, ni, nj))
-        lines.append("%s %s %d" % (_fmt(h), _fmt(h), len(patches)))
+            first = [int(round((a[d] - lo[d]) / h)) + 1 for d in range(dim)]
+            size = [int(round((b[d] - a[d]) / h)) for d in range(dim)]
+            patches.append(" ".join(str(v) for v in first + size))
+        lines.append(" ".join([_fmt(h)] * dim) + " %d" % len(patches))
         lines.extend(patches)
     return "\n".join(lines) + "\n"
+
+
+def _contraction_amr(lo, hi, h0, levels, boxes):
+    """2D form of amr_refined with boxes given as (xlo, xhi, ylo, yhi)."""
+    conv = lambda bs: [((x0, y0), (x1, y1)) for (x0, x1, y0, y1) in bs]
+    return amr_refined(lo, hi, h0, levels, (lambda lv: conv(boxes(lv))) if callable(boxes) else conv(boxes))
+
+
+def wall_layer_cavity(n=16, dim=3, levels=2, layer=0.05, Re=400.0, dt=2e-4, **kw):
+    """Lid-driven cavity on [0,1]^dim with base n^dim cells and `levels`
+    extra factor-2 levels in a layer of thickness `layer` along every wall
+    (each level's layer snapped outward to its parent grid).  Lid u = 1 on
+    the top wall (z = 1 in 3D), no-slip elsewhere, pressure Neumann."""
+    h0 = 1.0 / n
+
+    def boxes(lv):
+        hp = h0 / 2 ** (lv - 1)                      # parent spacing
+        t = min(0.5, math.ceil(layer / hp - 1e-9) * hp)
+        out = []
+        for d in range(dim):
+            for side in (0, 1):
+                blo, bhi = [0.0] * dim, [1.0] * dim
+                if side == 0:
+                    bhi[d] = t
+                else:
+                    blo[d] = 1.0 - t
+                out.append((blo, bhi))
+        return out
+    c = cavity(n=n, dim=dim, Re=Re, dt=dt, **kw)
+    c["domains"] = [amr_refined([0.0] * dim, [1.0] * dim, h0, levels, boxes)]
+    c["name"] = "cavity%dd_%d_wall%d" % (dim, n, levels)
+    c["ncells"] = None
+    return c
 
 
 def contraction(levels=3, h0=0.1, length=20.0, H=4.0, h=1.0, radius=1.0, Re=0.01, Wi=1.0,
@@ -216,3 +250,62 @@ def contraction(levels=3, h0=0.1, length=20.0, H=4.0, h=1.0, radius=1.0, Re=0.01
     params = default_params(Re=Re, dt=dt, De=Wi, beta=beta, **kw)
     return dict(dim=2, domains=[up, dn], bcs=bcs, params=params,
                 name="contraction_%dlev%s_h%s" % (levels, "_graded" if graded else "", f(h0)), ncells=None)
+
+
+# ---------------------------------------------------------------- sweeps
+class Pcg32:
+    """PCG32 (O'Neill, "PCG: A Family of Simple Fast Space-Efficient
+    Statistically Good Algorithms for Random Number Generation", 2014;
+    pcg_basic.c): 64-bit LCG state, XSH-RR output."""
+    MULT = 6364136223846793005
+    MASK = (1 << 64) - 1
+
+    def __init__(self, initstate=0, initseq=0):
+        self.state = 0
+        self.inc = ((initseq << 1) | 1) & self.MASK
+        self.next()
+        self.state = (self.state + initstate) & self.MASK
+        self.next()
+
+    def next(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & self.MASK
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def bounded(self, bound):
+        # unbiased: reject the low 2^32 mod bound values (pcg32_boundedrand_r)
+        threshold = ((1 << 32) - bound) % bound
+        while True:
+            r = self.next()
+            if r >= threshold:
+                return r % bound
+
+    def uniform(self, a, b):
+        return a + (b - a) * (self.next() / 4294967296.0)
+
+
+SWEEP_SIDES = (32, 48, 64, 96, 128, 192, 256)
+
+
+def sweep_table(count=64, seed=0):
+    """Draws, in order per simulation, the side N uniformly from
+    SWEEP_SIDES, Re = exp(U[ln 1, ln 100]) and Wi = exp(U[ln 0.1, ln 2])
+    with PCG32(initstate=seed, initseq=0)."""
+    g = Pcg32(seed, 0)
+    out = []
+    for _ in range(count):
+        n = SWEEP_SIDES[g.bounded(len(SWEEP_SIDES))]
+        re = math.exp(g.uniform(math.log(1.0), math.log(100.0)))
+        wi = math.exp(g.uniform(math.log(0.1), math.log(2.0)))
+        out.append((n, re, wi))
+    return out
+
+
+def sweep(count=64, seed=0, beta=0.5, dt=5e-4, table=None, **kw):
+    """Independent 2D Oldroyd-B lid-driven cavities (log-conformation,
+    semi-implicit Euler) of a parameter sweep, as case dicts for a batch."""
+    kw.setdefault("tempdiscrtype", SEMI_IMPLICIT_EULER)
+    rows = table if table is not None else sweep_table(count, seed)
+    return [ve_cavity(n, Re=re, Wi=wi, beta=beta, dt=dt, **kw) for (n, re, wi) in rows]

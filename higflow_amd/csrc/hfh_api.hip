@@ -239,6 +239,7 @@ int hfh_slv_converged(hfh_slv *s) { return s ? s->st.converged : fail("null solv
 // ----------------------------------------------------------------- step
 struct hfh_solver {
     std::unique_ptr<hfh::FlowBase> f;
+    std::vector<std::unique_ptr<hfh::FlowBase>> members;  // batched runs
 };
 
 hfh_solver *hfh_create(int dim) {
@@ -259,6 +260,53 @@ hfh_solver *hfh_create(int dim) {
         return 0;
     });
     return s;
+}
+
+hfh_solver *hfh_create_host(int dim) {
+    if (dim != 2 && dim != 3) {
+        fail("hfh_create_host: dim must be 2 or 3");
+        return nullptr;
+    }
+    hfh_solver *s = new hfh_solver();
+    if (dim == 2) s->f.reset(new hfh::Flow<2>(true));
+    else s->f.reset(new hfh::Flow<3>(true));
+    return s;
+}
+
+hfh_solver *hfh_create_batch(int dim, hfh_solver **members, int n) {
+    if ((dim != 2 && dim != 3) || !members || n <= 0) {
+        fail("hfh_create_batch: dim must be 2 or 3 and at least one member given");
+        return nullptr;
+    }
+    for (int i = 0; i < n; i++)
+        if (!members[i] || !members[i]->f || members[i]->f->dim() != dim) {
+            fail("hfh_create_batch: null member or member of another dimension");
+            return nullptr;
+        }
+    hfh_solver *s = nullptr;
+    int rc = guard([&] {
+        // HFH_HOST_ONLY=1: merge the host tables only (inspection, no device)
+        const char *ho = std::getenv("HFH_HOST_ONLY");
+        const bool host_only = ho && ho[0] == '1';
+        int nd = 0;
+        if (!host_only && (hipGetDeviceCount(&nd) != hipSuccess || nd == 0))
+            throw std::runtime_error("no HIP device visible");
+        std::unique_ptr<hfh_solver> b(new hfh_solver());
+        if (dim == 2) b->f.reset(new hfh::Flow<2>(host_only));
+        else b->f.reset(new hfh::Flow<3>(host_only));
+        std::vector<hfh::FlowBase *> ms;
+        for (int i = 0; i < n; i++) ms.push_back(members[i]->f.get());
+        int r = b->f->setup_batch(ms);
+        if (r < 0) throw std::runtime_error("hfh_create_batch: members not set up");
+        for (int i = 0; i < n; i++) {
+            b->members.push_back(std::move(members[i]->f));
+            delete members[i];
+            members[i] = nullptr;
+        }
+        s = b.release();
+        return 0;
+    });
+    return rc < 0 ? nullptr : s;
 }
 
 void hfh_destroy(hfh_solver *s) { delete s; }
@@ -381,6 +429,41 @@ int hfh_solve_stats(hfh_solver *s, int which, int *info, double *resid) {
     if (!info || !resid) return fail("null output pointers");
     hfh::SolveStats st;
     if (s->f->solve_stats(which, &st) < 0) return fail("no such system");
+    info[0] = st.iters;
+    info[1] = st.outer;
+    info[2] = st.fallback;
+    info[3] = st.restarts;
+    *resid = st.resid;
+    return st.converged ? 1 : 0;
+}
+int hfh_batch_size(hfh_solver *s) {
+    HFH_SOLVER(s);
+    return s->f->batch_size();
+}
+int hfh_batch_field_size(hfh_solver *s, int member, int field, int comp) {
+    HFH_SOLVER(s);
+    return guard([&] { return s->f->member_field(member, field, comp, nullptr, false); });
+}
+int hfh_batch_get_field(hfh_solver *s, int member, int field, int comp, double *out) {
+    HFH_SOLVER(s);
+    if (!out) return fail("null output");
+    return guard([&] { return s->f->member_field(member, field, comp, out, false); });
+}
+int hfh_batch_set_field(hfh_solver *s, int member, int field, int comp, const double *in) {
+    HFH_SOLVER(s);
+    if (!in) return fail("null input");
+    return guard([&] { return s->f->member_field(member, field, comp, const_cast<double *>(in), true); });
+}
+int hfh_batch_bc_set_values(hfh_solver *s, int member, int kind, int i, const double *vals) {
+    HFH_SOLVER(s);
+    if (!vals) return fail("null values");
+    return guard([&] { return s->f->member_bc_set_values(member, kind, i, vals); });
+}
+int hfh_batch_solve_stats(hfh_solver *s, int member, int which, int *info, double *resid) {
+    HFH_SOLVER(s);
+    if (!info || !resid) return fail("null output pointers");
+    hfh::SolveStats st;
+    if (s->f->member_stats(member, which, &st) < 0) return fail("no such member or system");
     info[0] = st.iters;
     info[1] = st.outer;
     info[2] = st.fallback;

@@ -69,6 +69,15 @@ class FlowBase {
     virtual int probe(int which, int reps, double *avg_ms, long *bytes) = 0;
     virtual void sync() = 0;
     virtual int operator_csr(int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v) = 0;
+    // Batched runs: a Flow made by setup_batch steps independent member
+    // simulations (set up host-only, same time step and schemes; Re, De and
+    // beta may differ) in the same launches.  Member m's fields, boundary
+    // values and solve statistics go through these.
+    virtual int setup_batch(const std::vector<FlowBase *> &members) = 0;
+    virtual int batch_size() const = 0;
+    virtual int member_field(int m, int field, int comp, double *buf, bool set) = 0;
+    virtual int member_bc_set_values(int m, int kind, int i, const double *vals) = 0;
+    virtual int member_stats(int m, int which, SolveStats *out) const = 0;
 };
 
 template <int D>
@@ -132,6 +141,11 @@ class Flow : public FlowBase {
     int step(bool ve) override;
     int stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn, int *ids,
                 double *w, double *rhs) override;
+    int setup_batch(const std::vector<FlowBase *> &members) override;
+    int batch_size() const override { return (int)mem_.size(); }
+    int member_field(int m, int field, int comp, double *buf, bool set) override;
+    int member_bc_set_values(int m, int kind, int i, const double *vals) override;
+    int member_stats(int m, int which, SolveStats *out) const override;
     void set_time(double t) override { par_.t = t; }
     double time() const override { return par_.t; }
     int pinned() const override { return desing_ ? pin_ : -1; }
@@ -201,7 +215,7 @@ class Flow : public FlowBase {
     std::vector<double> bcval_;
 
     // state layout
-    int Nc_ = 0, Nf_[D] = {0};
+    int Nc_ = 0, Nf_[D] = {0}, ne_ = 0;
     int oU_[D], oUS_[D], oFU_[D], oP_, oDP_, oF_, oK_, oKN_, oS_, oDv_, oZero_, oPK_, oPcv_, oPmom_, nX_;
     Group gDu_, gK_, gCv_, gMom_, gDiv_, gGrad_;
     std::vector<int> momSlots_[D];
@@ -244,6 +258,32 @@ class Flow : public FlowBase {
     int hoff_[D + 1] = {};
     bool hbatched_ = false;
     SolveStats stats_, hstats_[D];
+
+    // ---------------------------------------------------------- batch
+    // A member's state vector is a sequence of regions (offset, components,
+    // length per component).  The batched state keeps each region
+    // component-major with the members side by side inside a component,
+    // each member's block padded to a multiple of 1024 where the region
+    // holds the unknowns of a solve (velocity, pressure), so every member
+    // is one segment of the block-diagonal systems.
+    struct Region {
+        int off, ncomp, len;
+        bool pad;
+    };
+    std::vector<Region> regions() const;
+    int remap(int m, int x) const;  // member m's state offset -> batched offset
+    int field_offset(int field, int comp) const;
+    std::vector<Flow<D> *> mem_;
+    std::vector<std::vector<Region>> mreg_;   // per member
+    std::vector<std::vector<int>> bmoff_;     // [region][member]: offset inside one component block
+    std::vector<int> bbase_, btot_;           // per region: batched start, component block length
+    std::vector<int> bcoff_;                  // per member: first boundary value
+    std::vector<int> pseg_, hseg_[D];         // member block starts (pressure, velocity component d)
+    std::vector<double> momRe_[D], vePar_;    // per-row Re; per-cell Re, De, beta
+    DevBuf<double> dMomRe_[D], dVePar_;
+    std::vector<SolveStats> mstats_;          // per member: pressure, then velocity components
+    void collect_member_stats();
+    void upload_device();
     hipEvent_t ev0_[ST_NSTAGE] = {}, ev1_[ST_NSTAGE] = {};
     bool timing_on_ = false;
     double ms_[ST_NSTAGE] = {0};
@@ -417,6 +457,7 @@ int Flow<D>::setup(const Params &p) {
     oF_ = o;
     o += sdF_->size();
     int ne = sdED_->size();
+    ne_ = ne;
     oK_ = o;
     o += NSYM * ne;
     oKN_ = o;
@@ -764,3 +805,4 @@ void Flow<D>::record_projection() {
 }  // namespace hfh
 
 #include "hfh_flow_impl.hpp"
+#include "hfh_batch.hpp"

@@ -148,6 +148,12 @@ void Flow<D>::finalize_layout() {
         }
     }
     if (host_only_) return;
+    upload_device();
+}
+
+// Device tables from the host tables (single run or merged batch).
+template <int D>
+void Flow<D>::upload_device() {
     // device upload
     std::vector<double> hx(nX_, 0.0);
     X_.alloc(nX_);
@@ -182,7 +188,10 @@ void Flow<D>::finalize_layout() {
         upload_group(dGrad_, gGrad_, 0, 0, true);
         dGrad_.y.upload(y.data(), y.size(), st_);
     }
-    psys_.set_matrix(Arp_, Aci_, Av_, st_, true);
+    for (int d = 0; d < D; d++)
+        if (!momRe_[d].empty()) dMomRe_[d].upload(momRe_[d].data(), momRe_[d].size(), st_);
+    if (!vePar_.empty()) dVePar_.upload(vePar_.data(), vePar_.size(), st_);
+    psys_.set_matrix(Arp_, Aci_, Av_, st_, true, pseg_);
     if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
         const char *ek = std::getenv("HFH_KRYLOV");
         hbatched_ = !(ek && std::strcmp(ek, "classic") == 0);
@@ -195,6 +204,8 @@ void Flow<D>::finalize_layout() {
             std::vector<double> v;
             for (int d = 0; d < D; d++) {
                 hoff_[d] = (int)rp.size() - 1;
+                // batched runs: one segment per member inside the component
+                for (size_t q = 1; q < hseg_[d].size(); q++) segs.push_back(hoff_[d] + hseg_[d][q]);
                 for (int i = 0; i < Nf_[d]; i++) {
                     for (int k = Hrp_[d][i]; k < Hrp_[d][i + 1]; k++) {
                         ci.push_back(Hci_[d][k] + hoff_[d]);
@@ -229,7 +240,7 @@ void Flow<D>::validate() const {
     for (int d = 0; d < D; d++)
         for (int s : momSlots_[d])
             if (s < 0 || s >= nX_) fail("momentum slot out of range");
-    int ne = sdED_->size();
+    int ne = ne_;
     for (size_t i = 0; i < veSlots_.size(); i++) {
         int k = (int)(i / (ne > 0 ? ne : 1));
         int s = veSlots_[i];
@@ -342,7 +353,7 @@ int Flow<D>::step(bool ve) {
     if (ve != (par_.flowtype == VISCOELASTIC)) return -4;
     if (bc_dirty_) refold_constants();
     double *X = X_.p;
-    int ne = sdED_->size();
+    int ne = ne_;
     if (ve) {
         tbegin(ST_VE);
         run_group(dDu_, 1.0, X, ST_VE);
@@ -364,6 +375,7 @@ int Flow<D>::step(bool ve) {
         vp.model = par_.ve_model;
         vp.convec = par_.ve_convec;
         vp.transform = par_.kernel_transform;
+        vp.par = vePar_.empty() ? nullptr : dVePar_.p;
         k_constitutive<D><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
         HFH_CHECK(hipMemcpyAsync(X + oK_, X + oKN_, sizeof(double) * NSYM * ne, hipMemcpyDeviceToDevice, st_));
         tend(ST_VE);
@@ -387,6 +399,7 @@ int Flow<D>::step(bool ve) {
         mp.semi = semi;
         mp.bout = semi ? (hbatched_ ? hbB_.p + hoff_[d] : hb_[d].p) : nullptr;
         mp.bcst = semi ? hcst_[d].p : nullptr;
+        mp.re = momRe_[d].empty() ? nullptr : dMomRe_[d].p;
         if (mp.nf > 0)
             k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[d].p, dMomFlags_[d].p, dMomDelta_[d].p,
                                                              X);
@@ -447,19 +460,25 @@ int Flow<D>::step(bool ve) {
     if (par_.projtype == INCREMENTAL) k_axpy_field<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, X, oP_, oDP_);
     tend(ST_PROJECT);
     par_.t += par_.dt;
+    if (!mem_.empty()) collect_member_stats();
     return stats_.converged ? 0 : 1;
 }
 
 // Launches one kernel of the step `reps` times back to back on the library
 // stream between two hipEvents (inputs as left by the last step):
-// Every probed kernel recomputes its outputs from inputs it does not write,
-// so repeated launches leave the state unchanged.
+// the probed step kernels recompute their outputs from inputs they do not
+// write, so repeated launches leave the state unchanged; the solver probes
+// change only the solver's work vectors, which the next solve rewrites.
 //   0 pressure SpMV + partial dot (one CG iteration's operator pass)
 //   1 fused momentum kernel, first velocity component
 //   2 momentum interpolation rows
 //   3 cell divergence rows
 //   4 projection rows
 //   5 fused constitutive kernel (viscoelastic cases)
+//   6 / 7 fused pressure BiCGStab launch B, FP64 / FP32
+//   8 plain copy of 64 MiB
+//   9 / 10 fused FP64 BiCGStab launch A / B of the semi-implicit velocity
+//     block system (of the pressure system when there is none)
 template <int D>
 int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
     if (!ready_ || host_only_ || reps <= 0) return -1;
@@ -505,6 +524,20 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
             float ms = psys_.k32.probe(reps, &by, st_);
+            *avg_ms = ms;
+            *bytes = by;
+            return 0;
+        }
+        case 9:
+        case 10: {
+            // the FP64 fused BiCGStab launches A (9) and B (10) of the
+            // semi-implicit velocity block system (the pressure system's
+            // when the step has none)
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            FusedKrylov<double> &k = hbatched_ ? hsysB_.k64 : psys_.k64;
+            if (k.n == 0) return -1;
+            float ms = which == 9 ? k.probe_a(reps, &by, st_) : k.probe(reps, &by, st_);
             *avg_ms = ms;
             *bytes = by;
             return 0;
@@ -555,7 +588,7 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
         default: {
             if (par_.flowtype != VISCOELASTIC) return -1;
             VeParams vp;
-            vp.ne = sdED_->size();
+            vp.ne = ne_;
             vp.nrowK = gK_.nrows();
             vp.oDv = oDv_;
             vp.oK = oK_;
@@ -590,8 +623,8 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
 // (symmetric storage, comp = i*D+j), 7 S, 8 Du
 template <int D>
 int Flow<D>::field_size(int field, int comp) {
-    if (!ready_) return -1;
-    int ne = sdED_->size();
+    if (!ready_ || !mem_.empty()) return -1;  // batched runs: per member
+    int ne = ne_;
     switch (field) {
     case 0:
     case 3:
@@ -637,22 +670,25 @@ static inline int sym_index(int D, int i, int j) {
 }
 
 template <int D>
+int Flow<D>::field_offset(int field, int comp) const {
+    switch (field) {
+    case 0: return oU_[comp];
+    case 1: return oP_;
+    case 2: return oDP_;
+    case 3: return oUS_[comp];
+    case 4: return oF_;
+    case 5: return oFU_[comp];
+    case 6: return oK_ + sym_index(D, comp / D, comp % D) * ne_;
+    case 7: return oS_ + comp * ne_;
+    default: return oDv_ + comp * ne_;
+    }
+}
+
+template <int D>
 int Flow<D>::get_field(int field, int comp, double *out) {
     int n = field_size(field, comp);
-    if (n < 0 || host_only_) return -1;
-    int off;
-    int ne = sdED_->size();
-    switch (field) {
-    case 0: off = oU_[comp]; break;
-    case 1: off = oP_; break;
-    case 2: off = oDP_; break;
-    case 3: off = oUS_[comp]; break;
-    case 4: off = oF_; break;
-    case 5: off = oFU_[comp]; break;
-    case 6: off = oK_ + sym_index(D, comp / D, comp % D) * ne; break;
-    case 7: off = oS_ + comp * ne; break;
-    default: off = oDv_ + comp * ne; break;
-    }
+    if (n < 0 || host_only_ || !mem_.empty()) return -1;
+    int off = field_offset(field, comp);
     HFH_CHECK(hipMemcpyAsync(out, X_.p + off, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
     return n;
@@ -661,23 +697,9 @@ int Flow<D>::get_field(int field, int comp, double *out) {
 template <int D>
 int Flow<D>::set_field(int field, int comp, const double *in) {
     int n = field_size(field, comp);
-    if (n < 0 || host_only_) return -1;
-    int off;
-    int ne = sdED_->size();
-    switch (field) {
-    case 0: off = oU_[comp]; break;
-    case 1: off = oP_; break;
-    case 2: off = oDP_; break;
-    case 3: off = oUS_[comp]; break;
-    case 4: off = oF_; break;
-    case 5: off = oFU_[comp]; break;
-    case 6:
-        if (comp / D > comp % D) return n;  // symmetric storage keeps the upper triangle
-        off = oK_ + sym_index(D, comp / D, comp % D) * ne;
-        break;
-    case 7: off = oS_ + comp * ne; break;
-    default: off = oDv_ + comp * ne; break;
-    }
+    if (n < 0 || host_only_ || !mem_.empty()) return -1;
+    if (field == 6 && comp / D > comp % D) return n;  // symmetric storage keeps the upper triangle
+    int off = field_offset(field, comp);
     HFH_CHECK(hipMemcpyAsync(X_.p + off, in, sizeof(double) * n, hipMemcpyHostToDevice, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
     return n;
@@ -686,7 +708,7 @@ int Flow<D>::set_field(int field, int comp, const double *in) {
 template <int D>
 int Flow<D>::stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn, int *ids,
                      double *w, double *rhs) {
-    if (!ready_) return -1;
+    if (!ready_ || !mem_.empty()) return -1;
     Domain<D> *dm = domain == 0 ? sdp_.get() : domain == 2 ? sdED_.get() : nullptr;
     if (domain == 1) {
         if (dim < 0 || dim >= D) return -1;

@@ -2,7 +2,10 @@
 // BiCGStab in two launches per iteration (hfh_krylov.hpp), or BiCGStab with
 // a smoothed-aggregation V-cycle as right preconditioner (hfh_amg.hpp).
 #pragma once
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 
 #include "hfh_amg.hpp"
 #include "hfh_krylov.hpp"
@@ -63,6 +66,12 @@ class FusedKrylov {
         pred = 0;
         S = KSeg();
         tune(st);
+        // defined contents before the first solve (a cached lane choice skips
+        // the timing launches that would otherwise have written them)
+        for (KBuf<T> *b : {&r, &rh, &v}) k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, b->p, T(0));
+        k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
+        k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
+        k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
         if (nseg > 1) {
             const int rpw = KNT / lanes;
             std::vector<int> ws(nb), wb(nseg + 1);
@@ -129,7 +138,64 @@ class FusedKrylov {
         return ms;
     }
 
+    // Times `reps` launches of BiCGStab launch A at iteration 1, the solve's
+    // longest launch: x += alpha D^-1 p + omega D^-1 s, r = s - omega t,
+    // p_new = r + beta (p - omega v), v_new = (A D^-1) p_new.  The launches
+    // run on the current s, t, p and v with fixed scalars (omega = 1,
+    // beta = alpha = 1/2, no breakdown) and write r, p2, v2 and, in place
+    // of the caller's x, the CG-only buffer q; call between solves only
+    // (the next solve's init rewrites all of them).  Returns the average ms
+    // and the algorithmic bytes of one launch: CSR structure and values;
+    // s, t, p, v, x, D^-1 and rh read once; x, r, p_new and v_new written
+    // once; the partials.
+    float probe_a(int reps, long *bytes, hipStream_t st) {
+        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+        const T slotv[4] = {T(1), T(1), T(2), T(1)};  // t.s, t.t, rh.s, rh.t
+        const int slot[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
+        for (int k = 0; k < 4; k++)
+            k_fill<T><<<nblocks(nb), BLOCK, 0, st>>>(nb, part.p + (size_t)slot[k] * nb, T(0));
+        std::vector<T> hk(KS_N, T(0));
+        hk[KS_RHO + 0] = T(1);
+        hk[KS_ALPHA + 0] = T(0.5);
+        if (hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(ks.p, hk.data(), KS_N * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("probe set-up failed");
+        for (int k = 0; k < 4; k++)
+            if (hipMemcpy(part.p + (size_t)slot[k] * nb, &slotv[k], sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+                throw std::runtime_error("probe set-up failed");
+        float ms = 0;
+        switch (lanes) {
+        case 1: ms = time_a<1>(reps, st); break;
+        case 2: ms = time_a<2>(reps, st); break;
+        case 4: ms = time_a<4>(reps, st); break;
+        default: ms = time_a<8>(reps, st); break;
+        }
+        long nnz = 0;
+        int last = 0;
+        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
+        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 11 * sizeof(T) +
+                 (long)nb * 6 * sizeof(T);
+        return ms;
+    }
+
   private:
+    template <int L>
+    float time_a(int reps, hipStream_t st) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            throw std::runtime_error("hipEventCreate failed");
+        (void)hipEventRecord(a, st);
+        for (int rep = 0; rep < reps; rep++)
+            k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), rp, ci, vhat.p, dinv.p, q.p, r.p, rh.p, s.p,
+                                               t.p, p.p, v.p, p2.p, v2.p, part.p, nb, ks.p, ki.p, KSeg());
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return reps > 0 ? ms / reps : 0.0f;
+    }
     static void up(KBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
         d.alloc(h.size());
         if (!h.empty() &&
@@ -208,17 +274,42 @@ class FusedKrylov {
         time_b<L>(2, st);
         return time_b<L>(20, st);
     }
-    // picks the lanes per row with the fastest SpMV launch
+    // picks the lanes per row with the fastest SpMV launch.  The choice is
+    // kept per matrix shape (rows, entries, segments, precision) for the
+    // life of the process, so solvers built later for the same system (a
+    // second run of the same case) sum in the same order and give the same
+    // iterates.
     void tune(hipStream_t st) {
-        const float tl[4] = {time_lanes<1>(st), time_lanes<2>(st), time_lanes<4>(st), time_lanes<8>(st)};
-        const int ll[4] = {1, 2, 4, 8};
+        static std::mutex mu;
+        static std::map<std::tuple<int, long, int, int>, int> chosen;
+        long nz = 0;
+        {
+            int last = 0;
+            if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nz = last;
+        }
+        const auto key = std::make_tuple(n, nz, nseg, (int)sizeof(T));
         int best = 1;
-        float tb = 1e30f;
-        for (int k = 0; k < 4; k++)
-            if (tl[k] < tb) {
-                tb = tl[k];
-                best = ll[k];
+        bool known = false;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            auto it = chosen.find(key);
+            if (it != chosen.end()) {
+                best = it->second;
+                known = true;
             }
+        }
+        if (!known) {
+            const float tl[4] = {time_lanes<1>(st), time_lanes<2>(st), time_lanes<4>(st), time_lanes<8>(st)};
+            const int ll[4] = {1, 2, 4, 8};
+            float tb = 1e30f;
+            for (int k = 0; k < 4; k++)
+                if (tl[k] < tb) {
+                    tb = tl[k];
+                    best = ll[k];
+                }
+            std::lock_guard<std::mutex> g(mu);
+            chosen[key] = best;
+        }
         if (const char *e = std::getenv("HFH_KRYLOV_LANES")) {
             int f = std::atoi(e);
             if (f == 1 || f == 2 || f == 4 || f == 8) best = f;

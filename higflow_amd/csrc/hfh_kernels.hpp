@@ -92,6 +92,7 @@ struct MomParams {
     int semi;            // semi-implicit Euler: write the Helmholtz right-hand side
     double *bout;        // semi: b[f] = explicit part + boundary constant
     const double *bcst;  // semi: boundary constant of each Helmholtz row
+    const double *re = nullptr;  // per-facet Reynolds number (batched runs), else Re
 };
 
 // Per-facet slot layout (produced by the recorder):
@@ -226,7 +227,7 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
         }
         diff += (ul - 2.0 * uf + ur) / (fd[d2] * fd[d2] * 1.0 * 1.0);
     }
-    diff /= P.Re;
+    diff /= P.re ? P.re[f] : P.Re;
     double rhs = 0.0;
     double pt = P.incremental ? dpdx : 0.0;
     if (P.semi) {  // hig-flow-step.c:815-824, hig-flow-step-viscoelastic.c:1137-1148

@@ -15,6 +15,9 @@ struct VeParams {
     int oDv, oK, oKN, oS;
     double dt, Re, De, beta, tol, galpha;
     int model, convec, transform;
+    // per-cell Re, De and beta (three arrays of ne; batched runs), else the
+    // scalars above
+    const double *par = nullptr;
 };
 
 template <int D>
@@ -233,6 +236,8 @@ __global__ void __launch_bounds__(BLOCK) k_constitutive(VeParams P, const int *_
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.ne) return;
     const int ne = P.ne;
+    const double Re = P.par ? P.par[c] : P.Re, De = P.par ? P.par[(size_t)ne + c] : P.De,
+                 beta = P.par ? P.par[2 * (size_t)ne + c] : P.beta;
     constexpr int NS = D * (D + 1) / 2;
     double Du[D][D], Kn[D][D], Kc[D][D];
     for (int i = 0; i < D; i++)
@@ -285,7 +290,7 @@ __global__ void __launch_bounds__(BLOCK) k_constitutive(VeParams P, const int *_
                 ok += Om[i][k] * Kn[k][j];
                 ko += Kn[i][k] * Om[k][j];
             }
-            RHS[i][j] = ok - ko + 2.0 * BB[i][j] + MM[i][j] / P.De;
+            RHS[i][j] = ok - ko + 2.0 * BB[i][j] + MM[i][j] / De;
         }
     double vl[D], vr[D], cd[D];
     unsigned fl = flags[c];
@@ -340,8 +345,8 @@ __global__ void __launch_bounds__(BLOCK) k_constitutive(VeParams P, const int *_
     const double fA = 1.0, a = 1.0, xi = 0.0;
     for (int i = 0; i < D; i++) {
         for (int j = 0; j < D; j++) {
-            double s = (1.0 - P.beta) * (fA * A[i][j]) / (P.Re * P.De * (1.0 - xi)) - (1.0 - P.beta) * Du[i][j] / P.Re;
-            if (i == j) s += -a * (1.0 - P.beta) / (P.Re * P.De * (1.0 - xi));
+            double s = (1.0 - beta) * (fA * A[i][j]) / (Re * De * (1.0 - xi)) - (1.0 - beta) * Du[i][j] / Re;
+            if (i == j) s += -a * (1.0 - beta) / (Re * De * (1.0 - xi));
             X[P.oS + (size_t)(i * D + j) * ne + c] = s;
         }
     }

@@ -102,6 +102,16 @@ def load():
     L.hfh_version.argtypes = []
     L.hfh_device_count.argtypes = []
     L.hfh_create.argtypes = [C.c_int]
+    L.hfh_create_host.restype = vp
+    L.hfh_create_host.argtypes = [C.c_int]
+    L.hfh_create_batch.restype = vp
+    L.hfh_create_batch.argtypes = [C.c_int, C.POINTER(vp), C.c_int]
+    L.hfh_batch_size.argtypes = [vp]
+    L.hfh_batch_field_size.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.hfh_batch_get_field.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.hfh_batch_set_field.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.hfh_batch_bc_set_values.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.hfh_batch_solve_stats.argtypes = [vp, C.c_int, C.c_int, ip, dp]
     _LIB = L
     return L
 
@@ -134,11 +144,11 @@ class Flow:
     """Device-resident HiG-Flow solver built from a case dict
     (``higflow_amd.cases``); mirrors higflow_solver_step(_viscoelastic)."""
 
-    def __init__(self, case, **param_over):
+    def __init__(self, case, host_only=False, **param_over):
         self.L = load()
         self.dim = case["dim"]
         self.case = case
-        self.h = self.L.hfh_create(self.dim)
+        self.h = (self.L.hfh_create_host if host_only else self.L.hfh_create)(self.dim)
         if not self.h:
             raise HipError("hfh_create: " + self.L.hfh_last_error().decode())
         for amr in case["domains"]:
@@ -274,6 +284,76 @@ class Flow:
         _check(self.L, self.L.hfh_operator_csr(self.h, which, C.byref(n), C.byref(nnz), C.byref(sym), _ip(rp),
                                                _ip(ci), _dp(v)), "hfh_operator_csr")
         return rp, ci, v, bool(sym.value)
+
+
+class Batch(Flow):
+    """Independent simulations (case dicts of one dimension, same time step
+    and schemes; Re, De and beta may differ) stepped together on the device:
+    each step stage is one launch for all of them and the linear systems are
+    solved block-diagonally with one stopping test per simulation.  Fields,
+    boundary values and solve statistics are per member."""
+
+    def __init__(self, case_list, **param_over):
+        self.L = load()
+        if not case_list:
+            raise ValueError("empty batch")
+        self.dim = case_list[0]["dim"]
+        self.cases = list(case_list)
+        # members set up side by side (the library releases the GIL; each
+        # member's tables are its own)
+        from concurrent.futures import ThreadPoolExecutor
+        workers = max(1, min(16, os.cpu_count() or 1, len(self.cases)))
+        with ThreadPoolExecutor(workers) as ex:
+            members = list(ex.map(lambda c: Flow(c, host_only=True, **param_over), self.cases))
+        arr = (C.c_void_p * len(members))(*[m.h for m in members])
+        self.h = self.L.hfh_create_batch(self.dim, arr, len(members))
+        if not self.h:
+            raise HipError("hfh_create_batch: " + self.L.hfh_last_error().decode())
+        for m in members:
+            m.h = None  # freed by hfh_create_batch
+        self.n = len(members)
+        self.ve = self.cases[0]["params"].get("flowtype", 0) == 3
+        self.params = members[0].params
+
+    def size(self, member, field, comp=0):
+        return _check(self.L, self.L.hfh_batch_field_size(self.h, member, field, comp), "hfh_batch_field_size")
+
+    def get(self, member, field, comp=0):
+        out = np.zeros(self.size(member, field, comp))
+        _check(self.L, self.L.hfh_batch_get_field(self.h, member, field, comp, _dp(out)), "hfh_batch_get_field")
+        return out
+
+    def set(self, member, field, comp, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self.size(member, field, comp):
+            raise ValueError("field size mismatch")
+        _check(self.L, self.L.hfh_batch_set_field(self.h, member, field, comp, _dp(v)), "hfh_batch_set_field")
+
+    def solve_stats(self, member, which=0):
+        info = (C.c_int * 4)()
+        r = C.c_double(0)
+        conv = _check(self.L, self.L.hfh_batch_solve_stats(self.h, member, which, info, C.byref(r)),
+                      "hfh_batch_solve_stats")
+        return dict(converged=conv, iters=info[0], outer=info[1], fallback=info[2], restarts=info[3],
+                    resid=r.value)
+
+    def member(self, m):
+        """A view of member m with the single-run field interface."""
+        return _Member(self, m)
+
+
+class _Member:
+    def __init__(self, batch, m):
+        self.b, self.m = batch, m
+
+    def size(self, field, comp=0):
+        return self.b.size(self.m, field, comp)
+
+    def get(self, field, comp=0):
+        return self.b.get(self.m, field, comp)
+
+    def set(self, field, comp, values):
+        return self.b.set(self.m, field, comp, values)
 
 
 class LinearSolver:

@@ -161,8 +161,11 @@ int hfh_set_timing(hfh_solver *s, int on);
 /* launches one step kernel `reps` times between hipEvents on the library
  * stream: 0 pressure SpMV (+dot), 1 momentum, 2 momentum interpolation rows,
  * 3 divergence rows, 4 projection rows, 5 constitutive, 6 / 7 the fused
- * pressure BiCGStab SpMV launch in FP64 / FP32, 8 a plain copy of 64 MiB.
- * Average ms per launch and the algorithmic bytes of one launch. */
+ * pressure BiCGStab SpMV launch in FP64 / FP32, 8 a plain copy of 64 MiB,
+ * 9 / 10 the fused FP64 BiCGStab launch A (update + SpMV) / launch B (SpMV)
+ * of the semi-implicit velocity block system (the pressure system's when
+ * the step has none).  Average ms per launch and the algorithmic bytes of
+ * one launch.  Call between steps. */
 int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
 int hfh_sync(hfh_solver *s);
 /* the assembled systems as CSR: which 0 pressure (the matrix slv_set_Ai
@@ -171,6 +174,33 @@ int hfh_sync(hfh_solver *s);
  * system of component d (hig-flow-step.c:789-885).  Call with null rp/ci/v
  * for the sizes; *sym is 1 when the matrix is symmetric. */
 int hfh_operator_csr(hfh_solver *s, int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v);
+
+/* ------------------------------------------------- 3. batched parameter sweeps
+ * No counterpart in the reference, which runs a sweep as one driver process
+ * per simulation (higflow/example2d_Oldroyd/ns-example-2d.c:465-471 is one
+ * run's time loop).  Independent simulations on meshes of different sizes
+ * advance together: one launch per step stage for all of them, and their
+ * linear systems solved block-diagonally with one stopping test per
+ * simulation (the same tolerances and iterates as a run of its own).
+ *
+ * hfh_create_host: a solver that builds its mesh, stencils and operator
+ * tables on the host only (hfh_add_*, hfh_setup and hfh_bc_set_values as
+ * usual; stepping fails).  hfh_create_batch takes n such members, all with
+ * the same time step, schemes and solver settings (Re, De and beta may
+ * differ), merges them into one device solver and frees the member handles
+ * (members[i] is set to NULL).  Step the batch with hfh_solver_step /
+ * hfh_solver_step_viscoelastic; hfh_probe, hfh_timing and hfh_sync work as
+ * for one run; fields, boundary values and solve statistics go per member
+ * through hfh_batch_*. */
+hfh_solver *hfh_create_host(int dim);
+hfh_solver *hfh_create_batch(int dim, hfh_solver **members, int n);
+int hfh_batch_size(hfh_solver *s);
+int hfh_batch_field_size(hfh_solver *s, int member, int field, int comp);
+int hfh_batch_get_field(hfh_solver *s, int member, int field, int comp, double *out);
+int hfh_batch_set_field(hfh_solver *s, int member, int field, int comp, const double *in);
+int hfh_batch_bc_set_values(hfh_solver *s, int member, int kind, int bcindex, const double *vals);
+/* as hfh_solve_stats, for one member's system */
+int hfh_batch_solve_stats(hfh_solver *s, int member, int which, int *info, double *resid);
 
 #ifdef __cplusplus
 }

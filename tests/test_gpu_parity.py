@@ -84,6 +84,31 @@ def test_cavity_3d_parity():
     compare(cases.cavity(8, dim=3, Re=100.0, **TIGHT), 3)
 
 
+def test_wall_layer_cavity_3d_parity():
+    # 3D lid-driven cavity, base 6^3 with 2 levels in a 0.05 wall layer
+    # (6,376 cells; the shape of the 128^3 wall-layer configuration), Re=400
+    compare(cases.wall_layer_cavity(6, dim=3, levels=2, Re=400.0, dt=2e-4, **TIGHT), 2)
+
+
+@pytest.mark.parametrize("force32", ["0", "1"])
+def test_wall_layer_cavity_3d_mixed_error(force32, monkeypatch):
+    # FP32 Poisson inner iterations in FP64 refinement to 1e-9 against the
+    # FP64 step on the same mesh (FP32 refinement forced on every solve, or
+    # chosen by time)
+    monkeypatch.setenv("HFH_MIXED_FP32", force32)
+    case = cases.wall_layer_cavity(8, dim=3, levels=2, Re=400.0, dt=2e-4, rtol=1e-9, atol=1e-9, maxit=50000)
+    a = native.Flow(case, precision=0)
+    b = native.Flow(case, precision=1, inner_rtol=1e-5)
+    for _ in range(5):
+        a.step()
+        b.step()
+    assert b.solve_stats(0)["converged"] == 1
+    uscale = max(np.abs(a.get(0, d)).max() for d in range(3))
+    for d in range(3):
+        assert np.abs(b.get(0, d) - a.get(0, d)).max() / uscale < 1e-6
+    assert np.linalg.norm(b.get(1) - a.get(1)) / np.linalg.norm(a.get(1)) < 1e-6
+
+
 def test_nonincremental_projection():
     compare(cases.cavity(16, projtype=cases.NON_INCREMENTAL, **TIGHT), 3)
 
