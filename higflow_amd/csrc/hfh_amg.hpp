@@ -134,10 +134,20 @@ inline int sa_aggregate(const HCsr &A, double theta, std::vector<int> &agg) {
         for (int j : cols)
             if (j != i && std::fabs(acc[j]) >= theta * dg[i] * dg[j]) S[i].push_back(j);
     }
+    // rows without off-diagonal entries (the padding between the blocks of
+    // a block-diagonal system, a pinned unknown's identity row) join no
+    // aggregate: the smoother alone handles them.  As singleton aggregates
+    // their smoothed prolongation would shrink them by (1 - w)^2 per level
+    // until the coarsest matrix is numerically singular.
     agg.assign(n, -1);
+    for (int i = 0; i < n; i++) {
+        bool iso = true;
+        for (int k = A.rp[i]; k < A.rp[i + 1] && iso; k++) iso = A.ci[k] == i;
+        if (iso) agg[i] = -2;
+    }
     int na = 0;
     for (int i = 0; i < n; i++) {
-        if (agg[i] >= 0) continue;
+        if (agg[i] != -1) continue;
         bool free = true;
         for (int j : S[i])
             if (agg[j] >= 0) {
@@ -150,7 +160,7 @@ inline int sa_aggregate(const HCsr &A, double theta, std::vector<int> &agg) {
         na++;
     }
     for (int i = 0; i < n; i++) {
-        if (agg[i] >= 0) continue;
+        if (agg[i] != -1) continue;
         int a = -1;
         for (int j : S[i])
             if (agg[j] >= 0) {
@@ -159,6 +169,8 @@ inline int sa_aggregate(const HCsr &A, double theta, std::vector<int> &agg) {
             }
         agg[i] = a >= 0 ? a : na++;
     }
+    for (int i = 0; i < n; i++)
+        if (agg[i] == -2) agg[i] = -1;
     return na;
 }
 
@@ -201,12 +213,15 @@ inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<
         HCsr Pt;
         Pt.nrow = A.nrow;
         Pt.ncol = na;
-        Pt.rp.resize(A.nrow + 1);
-        Pt.ci.resize(A.nrow);
-        Pt.v.assign(A.nrow, 1.0);
+        Pt.rp.assign(A.nrow + 1, 0);
+        Pt.ci.clear();
+        Pt.v.clear();
         for (int i = 0; i < A.nrow; i++) {
-            Pt.rp[i + 1] = i + 1;
-            Pt.ci[i] = agg[i];
+            if (agg[i] >= 0) {
+                Pt.ci.push_back(agg[i]);
+                Pt.v.push_back(1.0);
+            }
+            Pt.rp[i + 1] = (int)Pt.ci.size();
         }
         HCsr S = A;  // S = I - w D^-1 A
         for (int i = 0; i < A.nrow; i++)
@@ -261,7 +276,9 @@ inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<
 constexpr int AL = 4;     // lanes per row in the V-cycle kernels
 constexpr int ANT = 256;  // threads per workgroup
 
-__device__ inline bool amg_done(const int *ki) { return ki[KI_DONE] != 0; }
+// ki is null for block-diagonal systems (the blocks stop one by one; the
+// cycle runs until the host sees every block done)
+__device__ inline bool amg_done(const int *ki) { return ki && ki[KI_DONE] != 0; }
 
 template <typename T>
 __device__ inline T arow(int rb, int re, const int *__restrict__ ci, const T *__restrict__ v, const T *__restrict__ x) {
@@ -499,8 +516,8 @@ __global__ void __launch_bounds__(KNT) k_abcg_a(int n, int it, T *__restrict__ x
                                                 const T *__restrict__ s, const T *__restrict__ t,
                                                 const T *__restrict__ ph, const T *__restrict__ sh,
                                                 T *__restrict__ p, const T *__restrict__ v, T *__restrict__ part,
-                                                int nb, T *ks, int *ki) {
-    const KCtx c = kctx<T>(KSeg(), nb, ks, ki);
+                                                int nb, T *ks, int *ki, KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
     const bool own = (int)threadIdx.x < KNT / L && i < n;
     if (it == 0) {
@@ -564,8 +581,8 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
                                                 const int *__restrict__ rp, const int *__restrict__ ci,
                                                 const T *__restrict__ val, const T *__restrict__ ph,
                                                 const T *__restrict__ rh, T *__restrict__ v, T *__restrict__ part,
-                                                int nb, T *ks, int *ki) {
-    const KCtx c = kctx<T>(KSeg(), nb, ks, ki);
+                                                int nb, T *ks, int *ki, KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[2] = {KP_RR, KP_BB};
     __shared__ T pv[3];
     reduce_partials<T, 2>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
@@ -590,8 +607,9 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
 
 template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_abcg_c(int n, int it, const T *__restrict__ r, const T *__restrict__ v,
-                                                T *__restrict__ s, T *__restrict__ part, int nb, T *ks, int *ki) {
-    const KCtx c = kctx<T>(KSeg(), nb, ks, ki);
+                                                T *__restrict__ s, T *__restrict__ part, int nb, T *ks, int *ki,
+                                                KSeg S) {
+    const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[1] = {KP_RV};
     __shared__ T pv[2];
     reduce_partials<T, 1>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
@@ -610,8 +628,10 @@ template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_abcg_d(int n, int it, const int *__restrict__ rp, const int *__restrict__ ci,
                                                 const T *__restrict__ val, const T *__restrict__ sh,
                                                 const T *__restrict__ s, const T *__restrict__ rh,
-                                                T *__restrict__ t, T *__restrict__ part, int nb, const int *ki) {
+                                                T *__restrict__ t, T *__restrict__ part, int nb, const int *ki,
+                                                KSeg S) {
     __shared__ int s_done;
+    if (S.wseg) ki += (size_t)S.wseg[blockIdx.x] * KI_N;
     if (threadIdx.x == 0) s_done = ki[KI_DONE] + ki[KI_BRK + (it & 1)];
     const int i = krow<L>();
     const bool in = i < n;

@@ -258,6 +258,10 @@ class Flow : public FlowBase {
     int hoff_[D + 1] = {};
     bool hbatched_ = false;
     SolveStats stats_, hstats_[D];
+    // initial guesses extrapolated from the last two solutions (HFH_EXTRAP=0:
+    // the last solution, as the reference's warm start)
+    bool extrap_ = true, extrap_first_ = true;
+    DevBuf<double> hxPrev_, pPrev_;
 
     // ---------------------------------------------------------- batch
     // A member's state vector is a sequence of regions (offset, components,

@@ -188,6 +188,12 @@ void Flow<D>::upload_device() {
         upload_group(dGrad_, gGrad_, 0, 0, true);
         dGrad_.y.upload(y.data(), y.size(), st_);
     }
+    {
+        const char *ex = std::getenv("HFH_EXTRAP");
+        extrap_ = !(ex && std::strcmp(ex, "0") == 0);
+        extrap_first_ = true;
+        pPrev_.alloc(Nc_);
+    }
     for (int d = 0; d < D; d++)
         if (!momRe_[d].empty()) dMomRe_[d].upload(momRe_[d].data(), momRe_[d].size(), st_);
     if (!vePar_.empty()) dVePar_.upload(vePar_.data(), vePar_.size(), st_);
@@ -220,6 +226,7 @@ void Flow<D>::upload_device() {
             hsysB_.set_matrix(rp, ci, v, st_, false, segs);
             hbB_.alloc(hoff_[D]);
             hxB_.alloc(hoff_[D]);
+            hxPrev_.alloc(hoff_[D]);
             k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hbB_.p, 0.0);
             k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hxB_.p, 0.0);
         } else {
@@ -410,7 +417,9 @@ int Flow<D>::step(bool ve) {
         bool spd = true;
         for (int d = 0; d < D; d++) {
             spd = spd && Hsym_[d];
-            k_copy<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], X + oUS_[d], hxB_.p + hoff_[d]);
+            k_extrap<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], X + oUS_[d], hxPrev_.p + hoff_[d],
+                                                                 hxB_.p + hoff_[d],
+                                                                 (extrap_first_ || !extrap_) ? 1 : 0);
         }
         if (par_.precision == 1)
             hsysB_.fused_mixed(spd, hbB_.p, hxB_.p, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
@@ -443,6 +452,7 @@ int Flow<D>::step(bool ve) {
     tend(ST_DIV);
     tbegin(ST_SOLVE);
     double *x = X + (par_.projtype == INCREMENTAL ? oDP_ : oP_);
+    if (extrap_) k_extrap<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, x, pPrev_.p, x, extrap_first_ ? 1 : 0);
     if (psys_.fused && par_.precision == 1) {
         stats_ = psys_.fused_mixed(symmetric_, bvec_.p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
     } else if (psys_.fused) {
@@ -460,6 +470,7 @@ int Flow<D>::step(bool ve) {
     if (par_.projtype == INCREMENTAL) k_axpy_field<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, X, oP_, oDP_);
     tend(ST_PROJECT);
     par_.t += par_.dt;
+    extrap_first_ = false;
     if (!mem_.empty()) collect_member_stats();
     return stats_.converged ? 0 : 1;
 }
@@ -700,6 +711,7 @@ int Flow<D>::set_field(int field, int comp, const double *in) {
     if (n < 0 || host_only_ || !mem_.empty()) return -1;
     if (field == 6 && comp / D > comp % D) return n;  // symmetric storage keeps the upper triangle
     int off = field_offset(field, comp);
+    extrap_first_ = true;
     HFH_CHECK(hipMemcpyAsync(X_.p + off, in, sizeof(double) * n, hipMemcpyHostToDevice, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
     return n;

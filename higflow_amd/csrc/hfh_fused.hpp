@@ -89,16 +89,19 @@ class FusedKrylov {
             S.wbeg = d_wbeg.p;
         }
         amg.ready = false;
-        if (want_amg && nseg == 1 && amg.build(n, hrp, hci, hv, st)) {
+        if (want_amg && amg.build(n, hrp, hci, hv, st)) {
             ph.alloc(n);
             sh.alloc(n);
         }
     }
     bool amg_on() const { return amg.ready; }
 
-    // r = b - A x and the flags; x is the initial guess
+    // r = b - A x and the flags; x is the initial guess.  `cg` asks for
+    // Jacobi-PCG (symmetric positive definite systems); with a V-cycle the
+    // solve is AMG-BiCGStab in either case.
     template <typename TB>
     void init(bool cg, const TB *b, T *x, hipStream_t st) {
+        cg = cg && !amg.ready;
         switch (lanes) {
         case 1: launch_init<TB, 1>(cg, b, x, st); break;
         case 2: launch_init<TB, 2>(cg, b, x, st); break;
@@ -108,6 +111,7 @@ class FusedKrylov {
     }
     // iterations [it0, it0 + count)
     void iterate(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
+        cg = cg && !amg.ready;
         switch (lanes) {
         case 1: launch_iter<1>(cg, it0, count, maxit, rtol, atol, x, st); break;
         case 2: launch_iter<2>(cg, it0, count, maxit, rtol, atol, x, st); break;
@@ -214,16 +218,19 @@ class FusedKrylov {
     }
     template <int L>
     void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
-        if (!cg && amg.ready) {
+        if (amg.ready) {
+            // one segment: the cycle stops with the solve; several: it runs
+            // until the host sees every segment done
+            const int *kc = nseg == 1 ? ki.p : nullptr;
             for (int it = it0; it < it0 + count; it++) {
                 k_abcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, s.p, t.p, ph.p, sh.p, p.p, v.p, part.p, nb, ks.p,
-                                                   ki.p);
-                amg.apply(p.p, ph.p, ki.p, st);
+                                                   ki.p, S);
+                amg.apply(p.p, ph.p, kc, st);
                 k_abcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, ph.p, rh.p, v.p, part.p,
-                                                   nb, ks.p, ki.p);
-                k_abcg_c<T, L><<<nb, KNT, 0, st>>>(n, it, r.p, v.p, s.p, part.p, nb, ks.p, ki.p);
-                amg.apply(s.p, sh.p, ki.p, st);
-                k_abcg_d<T, L><<<nb, KNT, 0, st>>>(n, it, rp, ci, val.p, sh.p, s.p, rh.p, t.p, part.p, nb, ki.p);
+                                                   nb, ks.p, ki.p, S);
+                k_abcg_c<T, L><<<nb, KNT, 0, st>>>(n, it, r.p, v.p, s.p, part.p, nb, ks.p, ki.p, S);
+                amg.apply(s.p, sh.p, kc, st);
+                k_abcg_d<T, L><<<nb, KNT, 0, st>>>(n, it, rp, ci, val.p, sh.p, s.p, rh.p, t.p, part.p, nb, ki.p, S);
             }
             return;
         }

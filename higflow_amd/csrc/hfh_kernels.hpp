@@ -261,6 +261,18 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
     X[P.us_off + f] = uf + P.dt * rhs;
 }
 
+// Initial guess of a time step's solve from the last two solutions:
+// out = 2 cur - prev (out = cur when `first`), then prev = cur.  `out` may
+// alias `cur`.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_extrap(int n, const T *cur, T *prev, T *out, int first) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T v = cur[i];
+    out[i] = first ? v : T(2) * v - prev[i];
+    prev[i] = v;
+}
+
 // p += dp over all cells (hig-flow-step.c:311-340)
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_axpy_field(int n, T *X, int dst, int src) {
