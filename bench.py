@@ -1,7 +1,7 @@
 #!/usr/bin/env python
 """Benchmark of the HiG-Flow projection time step on one MI355X.
 
-Metric (BASELINE.json): cell-updates/s per full time step, mixed precision
+Metric: cell-updates/s per full time step, mixed precision
 versus the plain FP64 GPU step, plus the Poisson solve time per step.
 ``value`` is the mixed-precision step's cell-updates/s (cells x steps / wall
 seconds of the timed steps, fields resident in HBM); the FP64 step, their
