@@ -96,18 +96,40 @@ __device__ inline KCtx kctx(const KSeg &S, int nb, T *&ks, int *&ki) {
 // then a butterfly) plus the done flag (slot K) and leaves the totals in
 // `sh`; the other waves go on with their row gathers meanwhile.  Every
 // workgroup computes the same totals.  Read them after kblock_sync().
+// The loads of a chunk (C partials of every slot per lane, 8 values in
+// flight) are issued together before any is summed, so the wave waits for
+// one memory round trip per chunk rather than one per partial; the sums
+// keep the order of a plain loop over the partials.
 template <typename T, int K>
 __device__ inline void reduce_partials(const T *part, int nb, const KCtx &c, const int (&slot)[K], const int *ki,
                                        T *sh, int brk = -1) {
     if (threadIdx.x >= 64) return;
     const int lane = threadIdx.x;
+    constexpr int C = K >= 8 ? 1 : 8 / K;
+    T a[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) a[k] = 0;
+#pragma unroll 1
+    for (int i0 = c.w0 + lane; i0 < c.w1; i0 += 64 * C) {
+        T v[K][C];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int q = 0; q < C; q++) {
+                const int i = i0 + q * 64;
+                v[k][q] = i < c.w1 ? part[(size_t)slot[k] * nb + i] : T(0);
+            }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int q = 0; q < C; q++)
+                if (i0 + q * 64 < c.w1) a[k] += v[k][q];
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        T a = 0;
-#pragma unroll 1
-        for (int i = c.w0 + lane; i < c.w1; i += 64) a += part[(size_t)slot[k] * nb + i];
-        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-        if (lane == 0) sh[k] = a;
+        T t = a[k];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) sh[k] = t;
     }
     if (lane == 0) sh[K] = (T)(ki[KI_DONE] + (brk >= 0 ? ki[brk] : 0));
 }
