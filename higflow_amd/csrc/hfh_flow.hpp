@@ -241,10 +241,17 @@ class Flow : public FlowBase {
     struct DGroup {
         DevBuf<int> rp, col, out, y;
         DevBuf<double> val, cst;
+        DevBuf<float> val32, cst32;  // FP32 rows (mixed form)
         int nrow = 0, ncomp = 1, in_stride = 0, out_stride = 0;
         int lanes = 1;  // lanes per row (from the mean row length)
-        bool has_y = false;
+        bool has_y = false, f32 = false;
     } dDu_, dK_, dCv_, dMom_, dDiv_, dGrad_;
+    // mixed form with HFH_MIXED_KERNELS=1: momentum, velocity-gradient and
+    // constitutive evaluations in FP32 arithmetic (FP64 state, eigen-
+    // decompositions, divergence and projection).  Off by default: at these
+    // sizes the step is latency-bound, and the FP32 right-hand sides cost
+    // solver iterations.
+    bool k32_ = false;
     DevBuf<int> dMomSlots_[D], dVeSlots_;
     DevBuf<unsigned> dMomFlags_[D], dVeFlags_;
     DevBuf<double> dMomDelta_[D], dVeDelta_;
@@ -336,9 +343,11 @@ class Flow : public FlowBase {
     void record_ve();
     void finalize_layout();
     void validate() const;
-    void upload_group(DGroup &dg, const Group &g, int pbase, int out_stride, bool has_y);
+    void upload_group(DGroup &dg, const Group &g, int pbase, int out_stride, bool has_y, bool f32 = false);
     void refold_constants();
     void run_group(DGroup &dg, double alpha, double *dst, int stage);
+    template <typename W>
+    void launch_rows(DGroup &dg, const W *val, const W *cst, W alpha, double *dst);
     void tbegin(int s) {
         if (!timing_on_) return;
         if (!ev0_[s]) {

@@ -169,10 +169,10 @@ void Flow<D>::upload_device() {
         dVeFlags_.upload(veFlags_.data(), veFlags_.size(), st_);
         dVeDelta_.upload(veDelta_.data(), veDelta_.size(), st_);
     }
-    upload_group(dDu_, gDu_, 0, 0, false);
-    upload_group(dK_, gK_, 0, gK_.nrows(), false);
-    upload_group(dCv_, gCv_, 0, 0, false);
-    upload_group(dMom_, gMom_, 0, 0, false);
+    upload_group(dDu_, gDu_, 0, 0, false, k32_);
+    upload_group(dK_, gK_, 0, gK_.nrows(), false, k32_);
+    upload_group(dCv_, gCv_, 0, 0, false, k32_);
+    upload_group(dMom_, gMom_, 0, 0, false, k32_);
     upload_group(dDiv_, gDiv_, 0, 0, false);
     // projection rows: u = us - dt * grad
     {
@@ -187,6 +187,10 @@ void Flow<D>::upload_device() {
         }
         upload_group(dGrad_, gGrad_, 0, 0, true);
         dGrad_.y.upload(y.data(), y.size(), st_);
+    }
+    {
+        const char *mk = std::getenv("HFH_MIXED_KERNELS");
+        k32_ = par_.precision == 1 && mk && std::strcmp(mk, "1") == 0;
     }
     {
         const char *ex = std::getenv("HFH_EXTRAP");
@@ -223,7 +227,9 @@ void Flow<D>::upload_device() {
                 segs.push_back((int)rp.size() - 1);
             }
             hoff_[D] = (int)rp.size() - 1;
-            hsysB_.set_matrix(rp, ci, v, st_, false, segs);
+            // HFH_VEL_AMG=1: AMG-BiCGStab for the velocity systems too
+            const char *va = std::getenv("HFH_VEL_AMG");
+            hsysB_.set_matrix(rp, ci, v, st_, va && std::strcmp(va, "1") == 0, segs);
             hbB_.alloc(hoff_[D]);
             hxB_.alloc(hoff_[D]);
             hxPrev_.alloc(hoff_[D]);
@@ -277,12 +283,13 @@ void Flow<D>::validate() const {
 }
 
 template <int D>
-void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool has_y) {
+void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool has_y, bool f32) {
     dg.nrow = g.nrows();
     dg.ncomp = g.ncomp;
     dg.in_stride = g.in_stride;
     dg.out_stride = out_stride;
     dg.has_y = has_y;
+    dg.f32 = f32;
     if (dg.nrow == 0) return;
     dg.rp.upload(g.rowptr.data(), g.rowptr.size(), st_);
     std::vector<int> col = g.col;
@@ -290,7 +297,14 @@ void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool
     std::vector<double> val = g.val;
     if (val.empty()) val.push_back(0.0);
     dg.col.upload(col.data(), col.size(), st_);
-    dg.val.upload(val.data(), val.size(), st_);
+    if (f32) {
+        std::vector<float> v32(val.begin(), val.end());
+        dg.val32.upload(v32.data(), v32.size(), st_);
+        HFH_CHECK(hipStreamSynchronize(st_));
+        dg.cst32.alloc(dg.nrow);
+    } else {
+        dg.val.upload(val.data(), val.size(), st_);
+    }
     dg.out.upload(g.out.data(), g.out.size(), st_);
     dg.cst.alloc(dg.nrow);
     // lanes per row from the mean row length (HFH_INTERP_LANES overrides)
@@ -311,7 +325,13 @@ void Flow<D>::refold_constants() {
     for (auto &p : gs) {
         if (p.d->nrow == 0) continue;
         std::vector<double> c = p.g->constants(bcval_);
-        p.d->cst.upload(c.data(), c.size(), st_);
+        if (p.d->f32) {
+            std::vector<float> c32(c.begin(), c.end());
+            p.d->cst32.upload(c32.data(), c32.size(), st_);
+            HFH_CHECK(hipStreamSynchronize(st_));
+        } else {
+            p.d->cst.upload(c.data(), c.size(), st_);
+        }
     }
     if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
         for (int d = 0; d < D; d++) {
@@ -332,16 +352,25 @@ template <int D>
 void Flow<D>::run_group(DGroup &dg, double alpha, double *dst, int stage) {
     if (dg.nrow == 0) return;
     (void)stage;
+    if (dg.f32)
+        launch_rows<float>(dg, dg.val32.p, dg.cst32.p, (float)alpha, dst);
+    else
+        launch_rows<double>(dg, dg.val.p, dg.cst.p, alpha, dst);
+}
+
+template <int D>
+template <typename W>
+void Flow<D>::launch_rows(DGroup &dg, const W *val, const W *cst, W alpha, double *dst) {
     long n = (long)dg.nrow * dg.ncomp;
     const int *y = dg.has_y ? dg.y.p : nullptr;
-#define HFH_INTERP_L(L)                                                                                      \
-    k_interp_l<double, L><<<nblocks(n * L), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, dg.val.p, \
-                                                             dg.cst.p, dg.out.p, y, alpha, dg.in_stride,     \
-                                                             dg.out_stride, X_.p, dst)
+#define HFH_INTERP_L(L)                                                                                        \
+    k_interp_l<double, L, W><<<nblocks(n * L), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, val, cst, \
+                                                                dg.out.p, y, alpha, dg.in_stride, dg.out_stride, \
+                                                                X_.p, dst)
     switch (dg.lanes) {
     case 1:
-        k_interp<double><<<nblocks(n), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, dg.val.p, dg.cst.p,
-                                                        dg.out.p, y, alpha, dg.in_stride, dg.out_stride, X_.p, dst);
+        k_interp<double, W><<<nblocks(n), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, val, cst, dg.out.p,
+                                                           y, alpha, dg.in_stride, dg.out_stride, X_.p, dst);
         break;
     case 2: HFH_INTERP_L(2); break;
     case 4: HFH_INTERP_L(4); break;
@@ -383,7 +412,10 @@ int Flow<D>::step(bool ve) {
         vp.convec = par_.ve_convec;
         vp.transform = par_.kernel_transform;
         vp.par = vePar_.empty() ? nullptr : dVePar_.p;
-        k_constitutive<D><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
+        if (k32_)
+            k_constitutive<D, float><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
+        else
+            k_constitutive<D><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
         HFH_CHECK(hipMemcpyAsync(X + oK_, X + oKN_, sizeof(double) * NSYM * ne, hipMemcpyDeviceToDevice, st_));
         tend(ST_VE);
     }
@@ -407,7 +439,10 @@ int Flow<D>::step(bool ve) {
         mp.bout = semi ? (hbatched_ ? hbB_.p + hoff_[d] : hb_[d].p) : nullptr;
         mp.bcst = semi ? hcst_[d].p : nullptr;
         mp.re = momRe_[d].empty() ? nullptr : dMomRe_[d].p;
-        if (mp.nf > 0)
+        if (mp.nf > 0 && k32_)
+            k_momentum<D, float><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[d].p, dMomFlags_[d].p,
+                                                                    dMomDelta_[d].p, X);
+        else if (mp.nf > 0)
             k_momentum<D><<<nblocks(mp.nf), BLOCK, 0, st_>>>(mp, dMomSlots_[d].p, dMomFlags_[d].p, dMomDelta_[d].p,
                                                              X);
     }

@@ -18,33 +18,35 @@ constexpr int BLOCK = 256;
 //   v = cst[r] + sum_k val[k] * X[col[k] + c*in_stride]
 //   dst[out[r] + c*out_stride] = alpha * v + (yoff ? X[yoff[r]] : 0)
 // One thread per (row, component); rows are short (identity/ghost rows) or
-// WLS rows of up to 45/120 entries kept contiguous.
-template <typename T>
+// WLS rows of up to 45/120 entries kept contiguous.  W is the precision of
+// the weights and of the arithmetic (the state T stays FP64): FP32 rows
+// serve the mixed form's momentum and constitutive evaluations.
+template <typename T, typename W = T>
 __global__ void __launch_bounds__(BLOCK) k_interp(int nrow, int ncomp, const int *__restrict__ rowptr,
-                                                  const int *__restrict__ col, const T *__restrict__ val,
-                                                  const T *__restrict__ cst, const int *__restrict__ out,
-                                                  const int *__restrict__ yoff, T alpha, int in_stride,
+                                                  const int *__restrict__ col, const W *__restrict__ val,
+                                                  const W *__restrict__ cst, const int *__restrict__ out,
+                                                  const int *__restrict__ yoff, W alpha, int in_stride,
                                                   int out_stride, const T *X, T *dst) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nrow * ncomp) return;
     int r = t % nrow, c = t / nrow;
     int b = rowptr[r], e = rowptr[r + 1];
-    T v = cst[r];
+    W v = cst[r];
     const T *Xc = X + (size_t)c * in_stride;
-    for (int k = b; k < e; k++) v += Xc[col[k]] * val[k];
-    T o = alpha * v;
-    if (yoff) o = X[yoff[r]] + o;
-    dst[out[r] + (size_t)c * out_stride] = o;
+    for (int k = b; k < e; k++) v += (W)Xc[col[k]] * val[k];
+    W o = alpha * v;
+    if (yoff) o = (W)X[yoff[r]] + o;
+    dst[out[r] + (size_t)c * out_stride] = (T)o;
 }
 
 // The same rows with L lanes per (row, component): lane l sums entries
 // l, l+L, ...; the lane sums combine by a butterfly in fixed order.  Long
 // WLS rows (up to 45/120 entries) then run L-way parallel.
-template <typename T, int L>
+template <typename T, int L, typename W = T>
 __global__ void __launch_bounds__(BLOCK) k_interp_l(int nrow, int ncomp, const int *__restrict__ rowptr,
-                                                    const int *__restrict__ col, const T *__restrict__ val,
-                                                    const T *__restrict__ cst, const int *__restrict__ out,
-                                                    const int *__restrict__ yoff, T alpha, int in_stride,
+                                                    const int *__restrict__ col, const W *__restrict__ val,
+                                                    const W *__restrict__ cst, const int *__restrict__ out,
+                                                    const int *__restrict__ yoff, W alpha, int in_stride,
                                                     int out_stride, const T *X, T *dst) {
     const long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) / L;
     const int lane = threadIdx.x % L;
@@ -52,24 +54,24 @@ __global__ void __launch_bounds__(BLOCK) k_interp_l(int nrow, int ncomp, const i
     const int r = in ? (int)(g % nrow) : 0, c = in ? (int)(g / nrow) : 0;
     const int b = in ? rowptr[r] : 0, e = in ? rowptr[r + 1] : 0;
     const T *Xc = X + (size_t)c * in_stride;
-    T v = 0;
+    W v = 0;
     // four entries per pass: indices first, then the gathered values
     constexpr int U = 4;
 #pragma unroll 1
     for (int k0 = b + lane; k0 < e; k0 += U * L) {
         int j[U];
-        T a[U];
+        W a[U];
         bool ok[U];
 #pragma unroll
         for (int q = 0; q < U; q++) {
             const int k = k0 + q * L;
             ok[q] = k < e;
             j[q] = ok[q] ? col[k] : 0;
-            a[q] = ok[q] ? val[k] : T(0);
+            a[q] = ok[q] ? val[k] : W(0);
         }
 #pragma unroll
         for (int q = 0; q < U; q++) {
-            const T x = Xc[j[q]];
+            const W x = (W)Xc[j[q]];
             if (ok[q]) v += x * a[q];
         }
     }
@@ -77,9 +79,9 @@ __global__ void __launch_bounds__(BLOCK) k_interp_l(int nrow, int ncomp, const i
     for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if (in && lane == 0) {
         v = cst[r] + v;
-        T o = alpha * v;
-        if (yoff) o = X[yoff[r]] + o;
-        dst[out[r] + (size_t)c * out_stride] = o;
+        W o = alpha * v;
+        if (yoff) o = (W)X[yoff[r]] + o;
+        dst[out[r] + (size_t)c * out_stride] = (T)o;
     }
 }
 
@@ -101,7 +103,7 @@ struct MomParams {
 //   velocities, then D tensor divergences (viscoelastic).
 // flags: bits 0-1 final convective type, bit 2+d2 infacet_rr, bit 2+D+d2
 // infacet_ll (hig-flow-discret.c:131-251).
-template <int D>
+template <int D, typename C = double>
 __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__restrict__ slots,
                                                     const unsigned *__restrict__ flags,
                                                     const double *__restrict__ fdelta, double *X) {
@@ -109,37 +111,37 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
     if (f >= P.nf) return;
     const int nf = P.nf;
     const int dim = P.dim;
-    const double a = 1.7500, b = 0.3750, c = 0.7500, d = 0.1250, e = 0.2500;
+    const C a = 1.7500, b = 0.3750, c = 0.7500, d = 0.1250, e = 0.2500;
     unsigned fl = flags[f];
     int ctype = fl & 3;
-    double fd[D];
-    for (int k = 0; k < D; k++) fd[k] = fdelta[(size_t)k * nf + f];
-    double uf = X[P.u_off + f];
-    double F = X[slots[f]];
-    double dpdx = X[slots[(size_t)nf + f]];
-    double conv = 0.0, diff = 0.0, tens = 0.0;
+    C fd[D];
+    for (int k = 0; k < D; k++) fd[k] = (C)fdelta[(size_t)k * nf + f];
+    C uf = (C)X[P.u_off + f];
+    C F = (C)X[slots[f]];
+    C dpdx = (C)X[slots[(size_t)nf + f]];
+    C conv = 0.0, diff = 0.0, tens = 0.0;
     int vk = 2 + 6 * D;
     for (int d2 = 0; d2 < D; d2++) {
         const int base = 2 + 6 * d2;
-        double ul = X[slots[(size_t)(base + 0) * nf + f]];
-        double ur = X[slots[(size_t)(base + 1) * nf + f]];
-        double v;
+        C ul = (C)X[slots[(size_t)(base + 0) * nf + f]];
+        C ur = (C)X[slots[(size_t)(base + 1) * nf + f]];
+        C v;
         if (d2 == dim) {
             v = uf;
         } else {
-            v = X[slots[(size_t)vk * nf + f]];
+            v = (C)X[slots[(size_t)vk * nf + f]];
             vk++;
         }
-        double dudxc = (ur - ul) / (2.0 * 1.0 * fd[d2]);
-        double dudxr = (ur - uf) / (1.0 * fd[d2]);
-        double dudxl = (uf - ul) / (1.0 * fd[d2]);
+        C dudxc = (ur - ul) / (C(2) * C(1) * fd[d2]);
+        C dudxr = (ur - uf) / (C(1) * fd[d2]);
+        C dudxl = (uf - ul) / (C(1) * fd[d2]);
         if (ctype == 2) {
             if (P.second == 1) {
-                double ull = X[slots[(size_t)(base + 3) * nf + f]];
-                double urr = X[slots[(size_t)(base + 2) * nf + f]];
+                C ull = (C)X[slots[(size_t)(base + 3) * nf + f]];
+                C urr = (C)X[slots[(size_t)(base + 2) * nf + f]];
                 int irr = (fl >> (2 + d2)) & 1, ill = (fl >> (2 + D + d2)) & 1;
-                double vc = X[slots[(size_t)(base + 4) * nf + f]];
-                double conv1 = 0.0, conv2 = 0.0, fi;
+                C vc = (C)X[slots[(size_t)(base + 4) * nf + f]];
+                C conv1 = 0.0, conv2 = 0.0, fi;
                 if (vc > 0.0) {
                     if (HFH_FLT_EQ(ur, ul)) {
                         conv1 = vc * uf;
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
                         }
                     }
                 }
-                vc = X[slots[(size_t)(base + 5) * nf + f]];
+                vc = (C)X[slots[(size_t)(base + 5) * nf + f]];
                 if (vc > 0.0) {
                     if (HFH_FLT_EQ(uf, ull)) {
                         conv2 = vc * ul;
@@ -197,51 +199,51 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
                 }
                 conv += ((conv1 - conv2) / fd[dim]);
             } else if (P.second == 2) {
-                double urr = X[slots[(size_t)(base + 2) * nf + f]];
-                double ull = X[slots[(size_t)(base + 3) * nf + f]];
-                double dudxrr = (urr - ur) / (1.0 * fd[d2]);
-                double dudxll = (uf - ull) / (1.0 * fd[d2]);
+                C urr = (C)X[slots[(size_t)(base + 2) * nf + f]];
+                C ull = (C)X[slots[(size_t)(base + 3) * nf + f]];
+                C dudxrr = (urr - ur) / (C(1) * fd[d2]);
+                C dudxll = (uf - ull) / (C(1) * fd[d2]);
                 (void)dudxll;
-                double dll = (ul - ull) / (1.0 * fd[d2]);
-                conv += (v + fabs(v)) * 0.1875 * dudxl;
-                conv += (v + fabs(v)) * 0.3750 * dudxr;
-                conv -= (v + fabs(v)) * 0.0625 * dudxrr;
-                conv += (v - fabs(v)) * 0.3750 * dudxl;
-                conv += (v - fabs(v)) * 0.1875 * dudxr;
-                conv -= (v - fabs(v)) * 0.0625 * dll;
+                C dll = (ul - ull) / (C(1) * fd[d2]);
+                conv += (v + fabs(v)) * C(0.1875) * dudxl;
+                conv += (v + fabs(v)) * C(0.3750) * dudxr;
+                conv -= (v + fabs(v)) * C(0.0625) * dudxrr;
+                conv += (v - fabs(v)) * C(0.3750) * dudxl;
+                conv += (v - fabs(v)) * C(0.1875) * dudxr;
+                conv -= (v - fabs(v)) * C(0.0625) * dll;
             } else {
-                double du2 = (ul - 2.0 * uf + ur) / (fd[d2] * fd[d2] * 1.0 * 1.0);
-                conv += 0.5 * (v + fabs(v)) * dudxl;
-                conv += 0.5 * (v - fabs(v)) * dudxr;
+                C du2 = (ul - C(2) * uf + ur) / (fd[d2] * fd[d2] * C(1) * C(1));
+                conv += C(0.5) * (v + fabs(v)) * dudxl;
+                conv += C(0.5) * (v - fabs(v)) * dudxr;
                 if (v > 0.0) {
-                    if (fabs(dudxl) > 1.0e-8) conv += 0.5 * v * fd[d2] * du2 / dudxl;
+                    if (fabs(dudxl) > 1.0e-8) conv += C(0.5) * v * fd[d2] * du2 / dudxl;
                 } else {
-                    if (fabs(dudxr) > 1.0e-8) conv -= 0.5 * v * fd[d2] * du2 / dudxr;
+                    if (fabs(dudxr) > 1.0e-8) conv -= C(0.5) * v * fd[d2] * du2 / dudxr;
                 }
             }
         } else if (ctype == 1) {
-            conv += 0.5 * (v + fabs(v)) * dudxl;
-            conv += 0.5 * (v - fabs(v)) * dudxr;
+            conv += C(0.5) * (v + fabs(v)) * dudxl;
+            conv += C(0.5) * (v - fabs(v)) * dudxr;
         } else {
             conv += v * dudxc;
         }
-        diff += (ul - 2.0 * uf + ur) / (fd[d2] * fd[d2] * 1.0 * 1.0);
+        diff += (ul - C(2) * uf + ur) / (fd[d2] * fd[d2] * C(1) * C(1));
     }
-    diff /= P.re ? P.re[f] : P.Re;
-    double rhs = 0.0;
-    double pt = P.incremental ? dpdx : 0.0;
+    diff /= (C)(P.re ? P.re[f] : P.Re);
+    C rhs = 0.0;
+    C pt = P.incremental ? dpdx : C(0);
     if (P.semi) {  // hig-flow-step.c:815-824, hig-flow-step-viscoelastic.c:1137-1148
         rhs += F;
         rhs -= pt;
         if (P.ve) {
             int tk = 2 + 6 * D + (D - 1);
-            for (int d2 = 0; d2 < D; d2++) tens += X[slots[(size_t)(tk + d2) * nf + f]];
+            for (int d2 = 0; d2 < D; d2++) tens += (C)X[slots[(size_t)(tk + d2) * nf + f]];
             rhs += tens;
         }
         rhs -= conv;
-        rhs *= P.dt;
+        rhs *= (C)P.dt;
         rhs += uf;
-        P.bout[f] = rhs + P.bcst[f];
+        P.bout[f] = (double)(rhs + (C)P.bcst[f]);
         return;
     }
     if (!P.ve) {
@@ -251,14 +253,14 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
         rhs += F;
     } else {
         int tk = 2 + 6 * D + (D - 1);
-        for (int d2 = 0; d2 < D; d2++) tens += X[slots[(size_t)(tk + d2) * nf + f]];
+        for (int d2 = 0; d2 < D; d2++) tens += (C)X[slots[(size_t)(tk + d2) * nf + f]];
         rhs += F;
         rhs -= pt;
         rhs += tens;
         rhs -= conv;
         rhs += diff;
     }
-    X[P.us_off + f] = uf + P.dt * rhs;
+    X[P.us_off + f] = (double)(uf + (C)P.dt * rhs);
 }
 
 // Initial guess of a time step's solve from the last two solutions:

@@ -123,10 +123,11 @@ __device__ inline double kjac(const VeParams &P, double l) {
 }
 
 // hig-flow-step.c:1185-1358 (CUBISTA cell convection of one component)
-__device__ inline double cubista_cell(double vl, double vr, double kl, double kr, double kll, double krr,
-                                      double kc, int il, int ir, int ill, int irr, double cd) {
-    const double a = 1.7500, b = 0.3750, c = 0.7500, d = 0.1250, e = 0.2500;
-    double vbar, fi, conv1 = 0.0, conv2 = 0.0;
+template <typename C>
+__device__ inline C cubista_cell(C vl, C vr, C kl, C kr, C kll, C krr, C kc, int il, int ir, int ill, int irr,
+                                 C cd) {
+    const C a = 1.7500, b = 0.3750, c = 0.7500, d = 0.1250, e = 0.2500;
+    C vbar, fi, conv1 = 0.0, conv2 = 0.0;
     vbar = vr;
     if (vbar > 0.0) {
         if (HFH_FLT_EQ(kr, kl)) {
@@ -229,7 +230,7 @@ __device__ inline double cubista_cell(double vl, double vr, double kl, double kr
 // slots (slot-major, ne cells): 0..2D-1 (vl_d, vr_d); 2D+4d+{0..3} (kl, kr,
 // kll, krr) of dimension d, component 0 (component m at +m*nrowK).
 // flags: bit 4d+{0..3} = incell of kl, kr, kll, krr.
-template <int D>
+template <int D, typename C = double>
 __global__ void __launch_bounds__(BLOCK) k_constitutive(VeParams P, const int *__restrict__ slots,
                                                         const unsigned *__restrict__ flags,
                                                         const double *__restrict__ cdelta, double *X) {
@@ -292,44 +293,47 @@ __global__ void __launch_bounds__(BLOCK) k_constitutive(VeParams P, const int *_
             }
             RHS[i][j] = ok - ko + 2.0 * BB[i][j] + MM[i][j] / De;
         }
-    double vl[D], vr[D], cd[D];
+    // convection, the kernel update and the stress in C (the eigen-
+    // decompositions and Omega above stay FP64)
+    C vl[D], vr[D], cd[D];
     unsigned fl = flags[c];
     for (int d = 0; d < D; d++) {
-        vl[d] = X[slots[(size_t)(2 * d) * ne + c]];
-        vr[d] = X[slots[(size_t)(2 * d + 1) * ne + c]];
-        cd[d] = cdelta[(size_t)d * ne + c];
+        vl[d] = (C)X[slots[(size_t)(2 * d) * ne + c]];
+        vr[d] = (C)X[slots[(size_t)(2 * d + 1) * ne + c]];
+        cd[d] = (C)cdelta[(size_t)d * ne + c];
     }
     double Knew[D][D];
     int m = 0;
     for (int i = 0; i < D; i++)
         for (int j = i; j < D; j++, m++) {
-            double rhs = 0.0;
+            C rhs = 0.0;
+            const C kij = (C)Kn[i][j];
             if (P.convec == 0) {
-                double dK[D];
+                C dK[D];
                 for (int d = 0; d < D; d++) {
                     const int kb = 2 * D + 4 * d;
-                    double Kl2 = X[slots[(size_t)(kb + 0) * ne + c] + (size_t)m * P.nrowK];
-                    double Kr2 = X[slots[(size_t)(kb + 1) * ne + c] + (size_t)m * P.nrowK];
+                    C Kl2 = (C)X[slots[(size_t)(kb + 0) * ne + c] + (size_t)m * P.nrowK];
+                    C Kr2 = (C)X[slots[(size_t)(kb + 1) * ne + c] + (size_t)m * P.nrowK];
                     int il = (fl >> (4 * d)) & 1, ir = (fl >> (4 * d + 1)) & 1;
-                    if (il && ir) dK[d] = (Kr2 - Kl2) / (2.0 * 1.0 * cd[d]);
-                    else if (ir) dK[d] = (Kr2 - Kn[i][j]) / (1.0 * cd[d]);
-                    else dK[d] = (Kn[i][j] - Kl2) / (1.0 * cd[d]);
+                    if (il && ir) dK[d] = (Kr2 - Kl2) / (C(2) * C(1) * cd[d]);
+                    else if (ir) dK[d] = (Kr2 - kij) / (C(1) * cd[d]);
+                    else dK[d] = (kij - Kl2) / (C(1) * cd[d]);
                 }
-                for (int d = 0; d < D; d++) rhs -= (0.5 * (vl[d] + vr[d])) * dK[d];
+                for (int d = 0; d < D; d++) rhs -= (C(0.5) * (vl[d] + vr[d])) * dK[d];
             } else {
                 for (int d = 0; d < D; d++) {
                     const int kb = 2 * D + 4 * d;
-                    double kl = X[slots[(size_t)(kb + 0) * ne + c] + (size_t)m * P.nrowK];
-                    double kr = X[slots[(size_t)(kb + 1) * ne + c] + (size_t)m * P.nrowK];
-                    double kll = X[slots[(size_t)(kb + 2) * ne + c] + (size_t)m * P.nrowK];
-                    double krr = X[slots[(size_t)(kb + 3) * ne + c] + (size_t)m * P.nrowK];
+                    C kl = (C)X[slots[(size_t)(kb + 0) * ne + c] + (size_t)m * P.nrowK];
+                    C kr = (C)X[slots[(size_t)(kb + 1) * ne + c] + (size_t)m * P.nrowK];
+                    C kll = (C)X[slots[(size_t)(kb + 2) * ne + c] + (size_t)m * P.nrowK];
+                    C krr = (C)X[slots[(size_t)(kb + 3) * ne + c] + (size_t)m * P.nrowK];
                     unsigned f4 = fl >> (4 * d);
-                    rhs -= cubista_cell(vl[d], vr[d], kl, kr, kll, krr, Kn[i][j], f4 & 1, (f4 >> 1) & 1,
+                    rhs -= cubista_cell<C>(vl[d], vr[d], kl, kr, kll, krr, kij, f4 & 1, (f4 >> 1) & 1,
                                         (f4 >> 2) & 1, (f4 >> 3) & 1, cd[d]);
                 }
             }
-            rhs += RHS[i][j];
-            double kernel = Kn[i][j] + P.dt * rhs;
+            rhs += (C)RHS[i][j];
+            const double kernel = (double)(kij + (C)P.dt * rhs);
             X[P.oKN + (size_t)m * ne + c] = kernel;
             Knew[i][j] = kernel;
             Knew[j][i] = kernel;
@@ -345,9 +349,10 @@ __global__ void __launch_bounds__(BLOCK) k_constitutive(VeParams P, const int *_
     const double fA = 1.0, a = 1.0, xi = 0.0;
     for (int i = 0; i < D; i++) {
         for (int j = 0; j < D; j++) {
-            double s = (1.0 - beta) * (fA * A[i][j]) / (Re * De * (1.0 - xi)) - (1.0 - beta) * Du[i][j] / Re;
-            if (i == j) s += -a * (1.0 - beta) / (Re * De * (1.0 - xi));
-            X[P.oS + (size_t)(i * D + j) * ne + c] = s;
+            const C cb = (C)beta, cre = (C)Re, cde = (C)De;
+            C s = (C(1) - cb) * (C(fA) * (C)A[i][j]) / (cre * cde * (C(1) - C(xi))) - (C(1) - cb) * (C)Du[i][j] / cre;
+            if (i == j) s += -C(a) * (C(1) - cb) / (cre * cde * (C(1) - C(xi)));
+            X[P.oS + (size_t)(i * D + j) * ne + c] = (double)s;
         }
     }
     (void)NS;

@@ -216,6 +216,38 @@ def test_mixed_precision_amr_contraction(force32, monkeypatch):
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
 
 
+@pytest.mark.parametrize("which", ["contraction", "ve_cavity"])
+def test_mixed_fp32_step_kernels_error(which, monkeypatch):
+    # momentum, velocity-gradient and constitutive evaluations in FP32
+    # arithmetic (eigendecompositions, divergence and projection in FP64):
+    # velocity within 1e-6 of the FP64 step, pressure within 1e-5 (1.3e-6
+    # measured on the contraction), log-conformation within 1e-4 (it starts
+    # from zero, so early steps are relatively noisy)
+    monkeypatch.setenv("HFH_MIXED_KERNELS", "1")
+    if which == "contraction":
+        case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, Re=0.01, Wi=1.0, beta=1.0 / 9.0,
+                                 dt=1e-4, rtol=1e-10, atol=1e-10, maxit=50000)
+    else:
+        case = cases.ve_cavity(32, Re=1.0, Wi=1.0, beta=0.5, dt=5e-4, tempdiscrtype=cases.SEMI_IMPLICIT_EULER,
+                               rtol=1e-10, atol=1e-10, maxit=50000)
+    h64 = native.Flow(case)
+    hmx = native.Flow(case, precision=1, inner_rtol=1e-5)
+    for _ in range(10):
+        h64.step()
+        hmx.step()
+    for d in range(2):
+        a, b = hmx.get(0, d), h64.get(0, d)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+    a, b = hmx.get(1), h64.get(1)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5
+    for m in (0, 1, 3):
+        a, b = hmx.get(6, m), h64.get(6, m)
+        nb = np.linalg.norm(b)
+        assert np.linalg.norm(a - b) <= 1e-4 * max(nb, 1e-30), (m, np.linalg.norm(a - b) / max(nb, 1e-30))
+    # the same runs through the FP64 kernels differ at solver-tolerance level
+    assert np.abs(hmx.get(0, 0) - h64.get(0, 0)).max() > 0
+
+
 def test_reference_example_inputs():
     # the reference example's own input files (higflow/example2d_Newt/input),
     # loaded as a case, through the HIP step against the oracle
