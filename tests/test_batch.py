@@ -72,3 +72,24 @@ def test_batch_rejects_members_with_other_schemes(host_only):
     cs[1] = cases.ve_cavity(24, Re=10.0, Wi=1.0, beta=0.5, dt=1e-4, tempdiscrtype=cases.SEMI_IMPLICIT_EULER)
     with pytest.raises(native.HipError, match="differ"):
         native.Batch(cs)
+
+
+def test_batch_entry_points_reject_bad_arguments(host_only):
+    L = native.load()
+    C = native.C
+    assert L.hfh_create_batch(2, None, 0) is None
+    assert b"member" in L.hfh_last_error()
+    m2 = native.Flow(_members()[0], host_only=True)
+    m3 = native.Flow(cases.cavity(4, dim=3), host_only=True)
+    arr = (C.c_void_p * 2)(m2.h, m3.h)
+    assert L.hfh_create_batch(2, arr, 2) is None
+    assert b"dimension" in L.hfh_last_error()
+    # a solver that is not set up
+    raw = L.hfh_create_host(2)
+    arr = (C.c_void_p * 1)(raw)
+    assert L.hfh_create_batch(2, arr, 1) is None
+    L.hfh_destroy(raw)
+    # batch accessors on a single run fail cleanly
+    assert L.hfh_batch_size(m2.h) == 0
+    out = np.zeros(16)
+    assert L.hfh_batch_get_field(m2.h, 0, 1, 0, out.ctypes.data_as(C.POINTER(C.c_double))) < 0
