@@ -227,9 +227,7 @@ void Flow<D>::upload_device() {
                 segs.push_back((int)rp.size() - 1);
             }
             hoff_[D] = (int)rp.size() - 1;
-            // HFH_VEL_AMG=1: AMG-BiCGStab for the velocity systems too
-            const char *va = std::getenv("HFH_VEL_AMG");
-            hsysB_.set_matrix(rp, ci, v, st_, va && std::strcmp(va, "1") == 0, segs);
+            hsysB_.set_matrix(rp, ci, v, st_, false, segs);
             hbB_.alloc(hoff_[D]);
             hxB_.alloc(hoff_[D]);
             hxPrev_.alloc(hoff_[D]);
