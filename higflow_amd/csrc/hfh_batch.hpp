@@ -306,7 +306,7 @@ int Flow<D>::member_field(int m, int field, int comp, double *buf, bool set) {
     if (!buf || n == 0) return n;  // size query
     if (set && field == 6 && comp / D > comp % D) return n;  // upper triangle stored
     int off = remap(m, f.field_offset(field, comp));
-    if (set) extrap_first_ = true;
+    if (set) extrap_hist_ = 0;
     if (set)
         HFH_CHECK(hipMemcpyAsync(X_.p + off, buf, sizeof(double) * n, hipMemcpyHostToDevice, st_));
     else

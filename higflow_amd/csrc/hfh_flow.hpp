@@ -265,10 +265,11 @@ class Flow : public FlowBase {
     int hoff_[D + 1] = {};
     bool hbatched_ = false;
     SolveStats stats_, hstats_[D];
-    // initial guesses extrapolated from the last two solutions (HFH_EXTRAP=0:
-    // the last solution, as the reference's warm start)
-    bool extrap_ = true, extrap_first_ = true;
-    DevBuf<double> hxPrev_, pPrev_;
+    // initial guesses extrapolated from the last solutions: HFH_EXTRAP=0 the
+    // last solution (the reference's warm start), 1 linear, 2 quadratic in
+    // time (default); extrap_hist_ counts the solutions kept
+    int extrap_ = 1, extrap_hist_ = 0;
+    DevBuf<double> hxPrev_, hxPrev2_, pPrev_, pPrev2_;
 
     // ---------------------------------------------------------- batch
     // A member's state vector is a sequence of regions (offset, components,

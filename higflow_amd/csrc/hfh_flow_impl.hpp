@@ -194,9 +194,10 @@ void Flow<D>::upload_device() {
     }
     {
         const char *ex = std::getenv("HFH_EXTRAP");
-        extrap_ = !(ex && std::strcmp(ex, "0") == 0);
-        extrap_first_ = true;
+        extrap_ = ex ? std::max(0, std::min(2, std::atoi(ex))) : 2;
+        extrap_hist_ = 0;
         pPrev_.alloc(Nc_);
+        pPrev2_.alloc(Nc_);
     }
     for (int d = 0; d < D; d++)
         if (!momRe_[d].empty()) dMomRe_[d].upload(momRe_[d].data(), momRe_[d].size(), st_);
@@ -231,6 +232,7 @@ void Flow<D>::upload_device() {
             hbB_.alloc(hoff_[D]);
             hxB_.alloc(hoff_[D]);
             hxPrev_.alloc(hoff_[D]);
+            hxPrev2_.alloc(hoff_[D]);
             k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hbB_.p, 0.0);
             k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hxB_.p, 0.0);
         } else {
@@ -451,8 +453,8 @@ int Flow<D>::step(bool ve) {
         for (int d = 0; d < D; d++) {
             spd = spd && Hsym_[d];
             k_extrap<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], X + oUS_[d], hxPrev_.p + hoff_[d],
-                                                                 hxB_.p + hoff_[d],
-                                                                 (extrap_first_ || !extrap_) ? 1 : 0);
+                                                                 hxPrev2_.p + hoff_[d], hxB_.p + hoff_[d],
+                                                                 std::min(extrap_, extrap_hist_));
         }
         if (par_.precision == 1)
             hsysB_.fused_mixed(spd, hbB_.p, hxB_.p, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
@@ -485,7 +487,8 @@ int Flow<D>::step(bool ve) {
     tend(ST_DIV);
     tbegin(ST_SOLVE);
     double *x = X + (par_.projtype == INCREMENTAL ? oDP_ : oP_);
-    if (extrap_) k_extrap<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, x, pPrev_.p, x, extrap_first_ ? 1 : 0);
+    if (extrap_)
+        k_extrap<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, x, pPrev_.p, pPrev2_.p, x, std::min(extrap_, extrap_hist_));
     if (psys_.fused && par_.precision == 1) {
         stats_ = psys_.fused_mixed(symmetric_, bvec_.p, x, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
     } else if (psys_.fused) {
@@ -503,7 +506,7 @@ int Flow<D>::step(bool ve) {
     if (par_.projtype == INCREMENTAL) k_axpy_field<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, X, oP_, oDP_);
     tend(ST_PROJECT);
     par_.t += par_.dt;
-    extrap_first_ = false;
+    extrap_hist_ = std::min(extrap_hist_ + 1, 2);
     if (!mem_.empty()) collect_member_stats();
     return stats_.converged ? 0 : 1;
 }
@@ -744,7 +747,7 @@ int Flow<D>::set_field(int field, int comp, const double *in) {
     if (n < 0 || host_only_ || !mem_.empty()) return -1;
     if (field == 6 && comp / D > comp % D) return n;  // symmetric storage keeps the upper triangle
     int off = field_offset(field, comp);
-    extrap_first_ = true;
+    extrap_hist_ = 0;
     HFH_CHECK(hipMemcpyAsync(X_.p + off, in, sizeof(double) * n, hipMemcpyHostToDevice, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
     return n;

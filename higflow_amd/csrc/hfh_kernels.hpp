@@ -263,16 +263,17 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(MomParams P, const int *__re
     X[P.us_off + f] = (double)(uf + (C)P.dt * rhs);
 }
 
-// Initial guess of a time step's solve from the last two solutions:
-// out = 2 cur - prev (out = cur when `first`), then prev = cur.  `out` may
-// alias `cur`.
+// Initial guess of a time step's solve from the last solutions: `order`
+// 0 out = cur; 1 out = 2 cur - p1; 2 out = 3 cur - 3 p1 + p2 (polynomial
+// extrapolation in time); then p2 = p1, p1 = cur.  `out` may alias `cur`.
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_extrap(int n, const T *cur, T *prev, T *out, int first) {
+__global__ void __launch_bounds__(BLOCK) k_extrap(int n, const T *cur, T *p1, T *p2, T *out, int order) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const T v = cur[i];
-    out[i] = first ? v : T(2) * v - prev[i];
-    prev[i] = v;
+    const T v = cur[i], a = p1[i];
+    out[i] = order == 0 ? v : order == 1 ? T(2) * v - a : T(3) * (v - a) + p2[i];
+    p2[i] = a;
+    p1[i] = v;
 }
 
 // p += dp over all cells (hig-flow-step.c:311-340)

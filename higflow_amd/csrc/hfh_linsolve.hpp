@@ -381,7 +381,9 @@ class DeviceSystem {
     int fp64_solves_left = 0;  // mixed: solves to run in FP64 after an FP32 failure
     int max_restarts = 1;      // fused64: restarts on the true residual
     double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
-    int since_probe_ = 0;
+    // the slower form is re-timed every probe_every_ solves; the interval
+    // doubles (up to 1600) while the faster form keeps winning
+    int since_probe_ = 0, probe_every_ = 50;
     int nseg = 1;
     std::vector<int> seg_off;
     std::vector<SolveStats> seg_stats;  // last solve, per segment
@@ -452,6 +454,7 @@ class DeviceSystem {
         }
         t_mixed_ = t_fp64_ = -1;
         since_probe_ = 0;
+        probe_every_ = 50;
         {
             const char *em = std::getenv("HFH_MIXED_FP32");
             force32_ = em && std::strcmp(em, "1") == 0;
@@ -584,7 +587,8 @@ class DeviceSystem {
             for (auto &g : seg_stats) g.fallback = 2;
             return S;
         }
-        bool use32;
+        bool use32, probing = false;
+        const bool pref32 = t_mixed_ >= 0 && t_fp64_ >= 0 && t_mixed_ < t_fp64_;
         if (force32_)
             use32 = true;
         else if (t_fp64_ < 0)
@@ -592,10 +596,11 @@ class DeviceSystem {
         else if (t_mixed_ < 0)
             use32 = true;
         else {
-            use32 = t_mixed_ < t_fp64_;
-            if (++since_probe_ >= 50) {
+            use32 = pref32;
+            if (++since_probe_ >= probe_every_) {
                 use32 = !use32;
                 since_probe_ = 0;
+                probing = true;
             }
         }
         const auto t0 = std::chrono::steady_clock::now();
@@ -622,6 +627,7 @@ class DeviceSystem {
             t_mixed_ = 1e30;  // FP32 failed on this system: keep FP64
         else
             t_fp64_ = t_fp64_ < 0 ? dt : 0.5 * (t_fp64_ + dt);
+        if (probing) probe_every_ = (t_mixed_ < t_fp64_) == pref32 ? std::min(2 * probe_every_, 1600) : 50;
         return S;
     }
 
