@@ -159,6 +159,9 @@ void Flow<D>::upload_device() {
     X_.alloc(nX_);
     X_.upload(hx.data(), nX_, st_);
     bvec_.alloc(Nc_);
+    // the divergence rows write every cell's entry; the padding entries of a
+    // batch (between member blocks) are never written and must read zero
+    k_fill<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, bvec_.p, 0.0);
     for (int d = 0; d < D; d++) {
         dMomSlots_[d].upload(momSlots_[d].data(), momSlots_[d].size(), st_);
         dMomFlags_[d].upload(momFlags_[d].data(), momFlags_[d].size(), st_);
@@ -198,6 +201,8 @@ void Flow<D>::upload_device() {
         extrap_hist_ = 0;
         pPrev_.alloc(Nc_);
         pPrev2_.alloc(Nc_);
+        k_fill<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, pPrev_.p, 0.0);
+        k_fill<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, pPrev2_.p, 0.0);
     }
     for (int d = 0; d < D; d++)
         if (!momRe_[d].empty()) dMomRe_[d].upload(momRe_[d].data(), momRe_[d].size(), st_);
@@ -233,6 +238,8 @@ void Flow<D>::upload_device() {
             hxB_.alloc(hoff_[D]);
             hxPrev_.alloc(hoff_[D]);
             hxPrev2_.alloc(hoff_[D]);
+            k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hxPrev_.p, 0.0);
+            k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hxPrev2_.p, 0.0);
             k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hbB_.p, 0.0);
             k_fill<double><<<nblocks(hoff_[D]), BLOCK, 0, st_>>>(hoff_[D], hxB_.p, 0.0);
         } else {

@@ -105,3 +105,41 @@ def test_batch_fields_and_boundary_values():
         single.step()
     w = worst_rel(b.member(1), single, 2, True)
     assert max(w.values()) <= 1e-10, w
+
+
+def test_batch_3d_members_match_oracle():
+    # 3D members of different sizes and Reynolds numbers, semi-implicit Euler
+    cs = [cases.cavity(n, dim=3, Re=re, dt=1e-3, tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TIGHT)
+          for n, re in ((4, 10.0), (6, 50.0))]
+    b = native.Batch(cs)
+    oracles = [hfo.Oracle(c) for c in cs]
+    for _ in range(3):
+        b.step()
+        for o in oracles:
+            o.step()
+    for m, o in enumerate(oracles):
+        w = worst_rel(b.member(m), o, 3, False)
+        bad = {k: v for k, v in w.items() if not v <= 1e-10}
+        assert not bad, (m, bad)
+
+
+def test_batch_ignores_recycled_device_memory():
+    # the padding between member blocks is never written by the step: it must
+    # not depend on what a previous user left in the memory the batch gets
+    # (a solver whose state is NaN is freed just before)
+    junk = native.Flow(cases.cavity(48, dim=3, **TIGHT))
+    for f, c in [(0, d) for d in range(3)] + [(1, 0), (2, 0), (3, 0), (3, 1), (3, 2)]:
+        junk.set(f, c, np.full(junk.size(f, c), np.nan))
+    junk.close()
+    cs = [cases.cavity(n, dim=3, Re=re, dt=1e-3, tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TIGHT)
+          for n, re in ((4, 10.0), (6, 50.0))]
+    b = native.Batch(cs)
+    oracles = [hfo.Oracle(c) for c in cs]
+    for _ in range(3):
+        b.step()
+        for o in oracles:
+            o.step()
+    for m, o in enumerate(oracles):
+        w = worst_rel(b.member(m), o, 3, False)
+        bad = {k: v for k, v in w.items() if not v <= 1e-10}
+        assert not bad, (m, bad)

@@ -109,6 +109,18 @@ def test_wall_layer_cavity_3d_mixed_error(force32, monkeypatch):
     assert np.linalg.norm(b.get(1) - a.get(1)) / np.linalg.norm(a.get(1)) < 1e-6
 
 
+@pytest.mark.parametrize("model", [cases.OLDROYD_B, cases.GIESEKUS])
+def test_viscoelastic_cavity_3d_semi_implicit(model):
+    # 3D log-conformation (3x3 Jacobi eigendecomposition), semi-implicit Euler
+    case = cases.cavity(6, dim=3, Re=1.0, dt=5e-4, flowtype=cases.VISCOELASTIC, ve_model=model, kernel_transform=1,
+                        De=1.0, beta=0.5, tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TIGHT)
+    compare(case, 3)
+
+
+def test_cavity_3d_semi_implicit():
+    compare(cases.cavity(8, dim=3, Re=10.0, dt=1e-3, tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TIGHT), 3)
+
+
 def test_nonincremental_projection():
     compare(cases.cavity(16, projtype=cases.NON_INCREMENTAL, **TIGHT), 3)
 
