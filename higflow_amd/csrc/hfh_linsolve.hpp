@@ -306,6 +306,22 @@ __global__ void __launch_bounds__(BLOCK) k_add_conv(int n, TD *x, const TS *e) {
     if (i < n) x[i] = x[i] + (TD)e[i];
 }
 
+// block-diagonal refinement: b32 = s_g r and x += u_g e, with g the segment of
+// the BLOCK-row block (segments start on BLOCK boundaries); sc holds s_g for
+// g < nseg and u_g at nseg + g
+__global__ void __launch_bounds__(BLOCK) k_convert_seg(int n, const double *__restrict__ r, float *__restrict__ b32,
+                                                       const int *__restrict__ bseg, const double *__restrict__ sc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b32[i] = (float)(sc[bseg[blockIdx.x]] * r[i]);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_add_conv_seg(int n, double *__restrict__ x, const float *__restrict__ e,
+                                                        const int *__restrict__ bseg, const double *__restrict__ sc,
+                                                        int nseg) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + sc[nseg + bseg[blockIdx.x]] * (double)e[i];
+}
+
 // r = b - A x (FP64), partial r.r and b.b
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_residual(int n, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -381,6 +397,7 @@ class DeviceSystem {
     int fp64_solves_left = 0;  // mixed: solves to run in FP64 after an FP32 failure
     int max_restarts = 1;      // fused64: restarts on the true residual
     double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
+    int n32_ = 0, n64_ = 0;              // timed solves of each form
     // the slower form is re-timed every probe_every_ solves; the interval
     // doubles (up to 1600) while the faster form keeps winning
     int since_probe_ = 0, probe_every_ = 50;
@@ -392,6 +409,10 @@ class DeviceSystem {
     int off_ks_ = 64, off_sc_ = 256;
     size_t hpin_size_ = 0;
     bool force32_ = false;  // HFH_MIXED_FP32=1: always refine with FP32 solves
+    int pred32_[3] = {0, 0, 0};  // FP32 iterations of the 1st, 2nd, later corrections
+    DevBuf<int> bseg_;              // segment of each BLOCK-row block (nseg > 1)
+    DevBuf<double> segsc_;          // per-segment scale s_g and unscale u_g
+    std::vector<double> hsegsc_;
     DevBuf<double> xbak;
 
     DeviceSystem() = default;
@@ -446,6 +467,15 @@ class DeviceSystem {
             hpin_size_ = pin;
         }
         seg_stats.assign(nseg, SolveStats());
+        if (nseg > 1) {
+            std::vector<int> hb(nb, nseg - 1);
+            for (int g = 0; g < nseg; g++)
+                for (int w = seg_off[g] / BLOCK; w < std::min(nb, (seg_off[g + 1] + BLOCK - 1) / BLOCK); w++) hb[w] = g;
+            bseg_.upload(hb.data(), hb.size(), st);
+            HFH_CHECK(hipStreamSynchronize(st));  // hb is a local
+            segsc_.alloc(2 * nseg);
+            hsegsc_.assign(2 * nseg, 0.0);
+        }
         if ((int)xbak.n < n) xbak.alloc(n);
         if (const char *er = std::getenv("HFH_RESTARTS")) max_restarts = std::max(0, std::atoi(er));
         {
@@ -453,6 +483,7 @@ class DeviceSystem {
             k64.want_amg = k32.want_amg = amg && !(ea && std::strcmp(ea, "0") == 0);
         }
         t_mixed_ = t_fp64_ = -1;
+        n32_ = n64_ = 0;
         since_probe_ = 0;
         probe_every_ = 50;
         {
@@ -577,24 +608,18 @@ class DeviceSystem {
 
     // Mixed form: per system, FP32 refinement or the FP64 solve, whichever
     // took less wall time on its last solves (both are timed on the host
-    // clock; each solve ends with a host synchronisation).  The form not in
-    // use is re-timed every 50 solves.
+    // clock; each solve ends with a host synchronisation).  The first four
+    // solves alternate the forms; after that the form not in use is re-timed
+    // every 50 solves.  Block-diagonal systems refine all segments together,
+    // each against its own tolerance.
     SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                            int maxit, hipStream_t st) {
-        if (nseg > 1) {  // several blocks: the FP64 solve
-            SolveStats S = fused64(spd, b, x, rtol, atol, maxit, st);
-            S.fallback = 2;
-            for (auto &g : seg_stats) g.fallback = 2;
-            return S;
-        }
         bool use32, probing = false;
         const bool pref32 = t_mixed_ >= 0 && t_fp64_ >= 0 && t_mixed_ < t_fp64_;
         if (force32_)
             use32 = true;
-        else if (t_fp64_ < 0)
-            use32 = false;
-        else if (t_mixed_ < 0)
-            use32 = true;
+        else if ((n64_ < 2 || n32_ < 2) && t_mixed_ < 1e30)
+            use32 = n32_ < n64_;  // first solves: the forms alternate, FP64 first
         else {
             use32 = pref32;
             if (++since_probe_ >= probe_every_) {
@@ -619,14 +644,17 @@ class DeviceSystem {
         } else {
             S = fused64(spd, b, x, rtol, atol, maxit, st);
             S.fallback = 2;
+            for (auto &g : seg_stats) g.fallback = 2;
         }
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // the second solve of a form replaces the time of its first (which
+        // pays one-time costs: first launches, no iteration prediction yet)
         if (use32 && S.fallback == 0)
-            t_mixed_ = t_mixed_ < 0 ? dt : 0.5 * (t_mixed_ + dt);
+            t_mixed_ = n32_++ < 2 ? dt : 0.5 * (t_mixed_ + dt);
         else if (use32)
             t_mixed_ = 1e30;  // FP32 failed on this system: keep FP64
         else
-            t_fp64_ = t_fp64_ < 0 ? dt : 0.5 * (t_fp64_ + dt);
+            t_fp64_ = n64_++ < 2 ? dt : 0.5 * (t_fp64_ + dt);
         if (probing) probe_every_ = (t_mixed_ < t_fp64_) == pref32 ? std::min(2 * probe_every_, 1600) : 50;
         return S;
     }
@@ -646,23 +674,77 @@ class DeviceSystem {
         int hi[KI_N];
         double hs[KS_N];
         int outer = 0, inner_total = 0;
-        double bn = 0, rn = 0, tol = 0;
+        // stopping test on the FP64 residual: one norm for one system, one
+        // per segment for block-diagonal systems (all must pass)
+        double worst = 0, need = 1.0;  // need: residual reduction still required
+        bool all_conv = false;
+        std::vector<double> prr(nseg > 1 ? nb : 0), pbb(nseg > 1 ? nb : 0);
+        std::vector<SolveStats> gs(nseg);
         for (;;) {
             k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
-            k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
-            HFH_CHECK(hipMemcpyAsync(hpin_ + off_sc_, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
-            HFH_CHECK(hipStreamSynchronize(st));
-            std::memcpy(h, hpin_ + off_sc_, sizeof h);
-            bn = std::sqrt(h[S_BB]);
-            rn = std::sqrt(h[S_RR]);
-            tol = std::max(rtol * bn, atol);
-            if (rn <= tol || outer >= 50 || inner_total >= maxit || !std::isfinite(rn)) break;
-            k_convert<float, double><<<nb, BLOCK, 0, st>>>(n, r.p, b32.p);
+            if (nseg == 1) {
+                k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
+                HFH_CHECK(hipMemcpyAsync(hpin_ + off_sc_, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
+                HFH_CHECK(hipStreamSynchronize(st));
+                std::memcpy(h, hpin_ + off_sc_, sizeof h);
+                const double bn = std::sqrt(h[S_BB]), rn = std::sqrt(h[S_RR]);
+                const double tol = std::max(rtol * bn, atol);
+                all_conv = rn <= tol;
+                worst = bn > 0 ? rn / bn : rn;
+                need = rn > 0 ? tol / rn : 1.0;
+            } else {
+                // segments start on BLOCK-row boundaries
+                HFH_CHECK(hipMemcpyAsync(prr.data(), part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+                HFH_CHECK(hipMemcpyAsync(pbb.data(), part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+                HFH_CHECK(hipStreamSynchronize(st));
+                all_conv = true;
+                worst = 0;
+                for (int g = 0; g < nseg; g++) {
+                    double sr = 0, sb = 0;
+                    for (int w = seg_off[g] / BLOCK; w < std::min(nb, (seg_off[g + 1] + BLOCK - 1) / BLOCK); w++) {
+                        sr += prr[w];
+                        sb += pbb[w];
+                    }
+                    const double bn = std::sqrt(sb), rn = std::sqrt(sr), tol = std::max(rtol * bn, atol);
+                    gs[g].resid = bn > 0 ? rn / bn : rn;
+                    gs[g].converged = rn <= tol;
+                    // the correction of segment g solves A d = s_g r_g with an
+                    // absolute tolerance of 1: its residual ends below half
+                    // of tol_g (or inner_rtol of ||r_g||); a converged segment
+                    // gets s_g = 0 and no correction
+                    const double target = std::max(0.5 * tol, 1e-300);
+                    hsegsc_[g] = gs[g].converged ? 0.0 : 1.0 / target;
+                    hsegsc_[nseg + g] = gs[g].converged ? 0.0 : target;
+                    all_conv = all_conv && gs[g].converged;
+                    worst = std::isfinite(gs[g].resid) ? std::max(worst, gs[g].resid) : gs[g].resid;
+                    if (!std::isfinite(worst)) break;
+                }
+            }
+            if (all_conv || outer >= 50 || inner_total >= maxit || !std::isfinite(worst)) break;
+            if (nseg == 1) {
+                k_convert<float, double><<<nb, BLOCK, 0, st>>>(n, r.p, b32.p);
+            } else {
+                HFH_CHECK(hipMemcpyAsync(segsc_.p, hsegsc_.data(), 2 * nseg * sizeof(double), hipMemcpyHostToDevice,
+                                         st));
+                k_convert_seg<<<nb, BLOCK, 0, st>>>(n, r.p, b32.p, bseg_.p, segsc_.p);
+            }
             k_fill<float><<<nb, BLOCK, 0, st>>>(n, x32.p, 0.0f);
             k32.init<float>(spd, b32.p, x32.p, st);
             // an inner solve may take at most twice the FP64 solve's count
             const int cap = std::min(maxit - inner_total, std::max(200, 2 * k64.pred + 50));
-            const int used = run_fused<float>(k32, spd, cap, (float)inner_rtol, 0.0f, x32.p, hi, hs, st);
+            // one system: the inner solve reduces its residual by what is
+            // still needed (with a factor 2 to spare), never further than
+            // inner_rtol, so a last correction with one digit to gain is
+            // short; block-diagonal systems get the same per segment from
+            // the scaling above and an absolute tolerance of 1
+            const double eff = std::min(0.5, std::max(inner_rtol, 0.5 * need));
+            // first and later corrections differ in length: each keeps its
+            // own iteration prediction
+            const int slot = std::min(outer, 2);
+            k32.pred = pred32_[slot];
+            const int used = nseg == 1 ? run_fused<float>(k32, spd, cap, (float)eff, 0.0f, x32.p, hi, hs, st)
+                                       : run_fused<float>(k32, spd, cap, (float)inner_rtol, 1.0f, x32.p, hi, hs, st);
+            pred32_[slot] = k32.pred;
             inner_total += std::max(used, 1);
             if (!(hi[KI_DONE] && hi[KI_CONV])) {
                 // FP32 Krylov cannot reach inner_rtol on this system (too
@@ -675,13 +757,23 @@ class DeviceSystem {
                 F.fallback = 1;
                 return F;
             }
-            k_add_conv<double, float><<<nb, BLOCK, 0, st>>>(n, x, x32.p);
+            if (nseg == 1)
+                k_add_conv<double, float><<<nb, BLOCK, 0, st>>>(n, x, x32.p);
+            else
+                k_add_conv_seg<<<nb, BLOCK, 0, st>>>(n, x, x32.p, bseg_.p, segsc_.p, nseg);
             outer++;
         }
         S.iters = inner_total;
         S.outer = outer;
-        S.resid = bn > 0 ? rn / bn : rn;
-        S.converged = rn <= tol;
+        S.resid = worst;
+        S.converged = all_conv;
+        if (nseg > 1) {
+            for (auto &g : gs) {
+                g.iters = inner_total;
+                g.outer = outer;
+            }
+            seg_stats = gs;
+        }
         return S;
     }
 

@@ -123,6 +123,33 @@ def test_batch_3d_members_match_oracle():
         assert not bad, (m, bad)
 
 
+@pytest.mark.parametrize("force32", ["0", "1"])
+def test_batch_mixed_precision_error(force32, monkeypatch):
+    # mixed form on a batch: the block-diagonal systems are refined together
+    # (FP32 corrections, FP64 residuals), each member against its own
+    # tolerance; forced on every solve or chosen by timing
+    monkeypatch.setenv("HFH_MIXED_FP32", force32)
+    rows = [(16, 2.5, 0.4), (24, 20.0, 1.5), (32, 1.0, 0.8)]
+    tol = dict(rtol=1e-10, atol=1e-10, maxit=50000)
+    b64 = native.Batch(cases.sweep(table=rows, dt=5e-4, **tol))
+    bmx = native.Batch(cases.sweep(table=rows, dt=5e-4, **tol), precision=1, inner_rtol=1e-5)
+    for _ in range(6):
+        b64.step()
+        bmx.step()
+    for m in range(len(rows)):
+        for f, c in ((0, 0), (0, 1), (1, 0)):
+            a, b = bmx.get(m, f, c), b64.get(m, f, c)
+            assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b), (m, f, c)
+        for which in (0, 1, 2):
+            st = bmx.solve_stats(m, which)
+            assert st["converged"] == 1, (m, which, st)
+        if force32 == "1":
+            # the velocity systems (well conditioned) refine in FP32
+            for which in (1, 2):
+                st = bmx.solve_stats(m, which)
+                assert st["fallback"] == 0 and st["outer"] >= 1, (m, which, st)
+
+
 def test_batch_ignores_recycled_device_memory():
     # the padding between member blocks is never written by the step: it must
     # not depend on what a previous user left in the memory the batch gets
