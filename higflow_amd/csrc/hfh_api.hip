@@ -392,6 +392,22 @@ int hfh_get_field(hfh_solver *s, int field, int comp, double *out) {
     HFH_SOLVER(s);
     return guard([&] { return s->f->get_field(field, comp, out); });
 }
+int hfh_cell_bounds(hfh_solver *s, double *lohi) {
+    HFH_SOLVER(s);
+    if (!lohi) {
+        g_err = "hfh_cell_bounds: null output";
+        return -1;
+    }
+    return guard([&] { return s->f->cell_bounds(lohi); });
+}
+int hfh_field_range(hfh_solver *s, int field, int comp, double *vmin, double *vmax) {
+    HFH_SOLVER(s);
+    if (!vmin || !vmax) {
+        g_err = "hfh_field_range: null output";
+        return -1;
+    }
+    return guard([&] { return s->f->field_range(field, comp, vmin, vmax); });
+}
 int hfh_set_field(hfh_solver *s, int field, int comp, const double *in) {
     HFH_SOLVER(s);
     return guard([&] { return s->f->set_field(field, comp, in); });

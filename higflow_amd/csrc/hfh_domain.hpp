@@ -98,6 +98,14 @@ class Domain {
         if (fdim < 0) return trees[cells[lid].first]->center(cells[lid].second);
         return trees[facets[lid].first]->facet_center(facets[lid].second);
     }
+    // lowpoint and highpoint of cell lid (cell domains)
+    void bounds_of(int lid, double *lo, double *hi) const {
+        const auto &nd = trees[cells[lid].first]->n[cells[lid].second];
+        for (int i = 0; i < D; i++) {
+            lo[i] = nd.lo[i];
+            hi[i] = nd.hi[i];
+        }
+    }
     Vec<D> delta_of(int lid) const {
         if (fdim < 0) return trees[cells[lid].first]->delta(cells[lid].second);
         return trees[facets[lid].first]->delta(facets[lid].second.cell);

@@ -51,6 +51,10 @@ class FlowBase {
     virtual int field_centers(int field, int comp, double *xyz) = 0;
     virtual int get_field(int field, int comp, double *out) = 0;
     virtual int set_field(int field, int comp, const double *in) = 0;
+    // min and max of a field on the device (the reference's Vmin/Vmax log)
+    virtual int field_range(int field, int comp, double *mn, double *mx) = 0;
+    // lowpoint then highpoint of every pressure cell, in local-id order
+    virtual int cell_bounds(double *lohi) = 0;
     virtual int bc_count(int kind, int i) = 0;
     virtual int bc_centers(int kind, int i, double *xyz) = 0;
     virtual int bc_set_values(int kind, int i, const double *vals) = 0;
@@ -118,6 +122,13 @@ class Flow : public FlowBase {
     int field_centers(int field, int comp, double *xyz) override;
     int get_field(int field, int comp, double *out) override;
     int set_field(int field, int comp, const double *in) override;
+    int field_range(int field, int comp, double *mn, double *mx) override;
+    int cell_bounds(double *lohi) override {
+        if (!ready_ || !mem_.empty()) return -1;
+        const int n = field_size(1, 0);
+        for (int i = 0; i < n; i++) sdp_->bounds_of(i, lohi + (size_t)i * 2 * D, lohi + (size_t)i * 2 * D + D);
+        return n;
+    }
     int bc_count(int kind, int i) override {
         Boundary<D> *b = bc_of(kind, i);
         return b ? (int)b->cells.size() : -1;
@@ -298,6 +309,8 @@ class Flow : public FlowBase {
     void upload_device();
     hipEvent_t ev0_[ST_NSTAGE] = {}, ev1_[ST_NSTAGE] = {};
     bool timing_on_ = false;
+    bool verbose_ = false;  // HFH_VERBOSE=1: velocity range per step
+    DevBuf<double> mm_;     // k_minmax output
     double ms_[ST_NSTAGE] = {0};
     long calls_[ST_NSTAGE] = {0};
     long bytes_[ST_NSTAGE] = {0};

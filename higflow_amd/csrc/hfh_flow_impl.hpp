@@ -196,6 +196,12 @@ void Flow<D>::upload_device() {
         k32_ = par_.precision == 1 && mk && std::strcmp(mk, "1") == 0;
     }
     {
+        // HFH_VERBOSE=1: print each step's velocity range as the reference's
+        // final-velocity step does (hig-flow-step.c:302-306)
+        const char *ev = std::getenv("HFH_VERBOSE");
+        verbose_ = ev && std::strcmp(ev, "1") == 0;
+    }
+    {
         const char *ex = std::getenv("HFH_EXTRAP");
         extrap_ = ex ? std::max(0, std::min(2, std::atoi(ex))) : 2;
         extrap_hist_ = 0;
@@ -515,6 +521,13 @@ int Flow<D>::step(bool ve) {
     par_.t += par_.dt;
     extrap_hist_ = std::min(extrap_hist_ + 1, 2);
     if (!mem_.empty()) collect_member_stats();
+    if (verbose_ && mem_.empty())
+        for (int d = 0; d < D; d++) {
+            double mn, mx;
+            field_range(0, d, &mn, &mx);
+            std::printf("===> %d: Vmin = %15.10lf <===> Vmax = %15.10lf <===\n", d, mn, mx);
+            std::fflush(stdout);
+        }
     return stats_.converged ? 0 : 1;
 }
 
@@ -745,6 +758,22 @@ int Flow<D>::get_field(int field, int comp, double *out) {
     int off = field_offset(field, comp);
     HFH_CHECK(hipMemcpyAsync(out, X_.p + off, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
+    return n;
+}
+
+// min and max over the unknowns of a field (for velocity: the reference's
+// Vmin/Vmax over the unknown facets, hig-flow-step.c:250-306)
+template <int D>
+int Flow<D>::field_range(int field, int comp, double *mn, double *mx) {
+    int n = field_size(field, comp);
+    if (n < 0 || host_only_ || !mem_.empty()) return -1;
+    if (!mm_.p) mm_.alloc(2);
+    k_minmax<<<1, 1024, 0, st_>>>(n, X_.p + field_offset(field, comp), mm_.p);
+    double h[2];
+    HFH_CHECK(hipMemcpyAsync(h, mm_.p, sizeof h, hipMemcpyDeviceToHost, st_));
+    HFH_CHECK(hipStreamSynchronize(st_));
+    *mn = h[0];
+    *mx = h[1];
     return n;
 }
 

@@ -297,4 +297,36 @@ __global__ void __launch_bounds__(BLOCK) k_fill(int n, T *x, T v) {
 
 inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
+// out[0] = min, out[1] = max of x[0, n), starting from +-1e16 as the
+// reference's final-velocity loop does (hig-flow-step.c:250-251, 290-291);
+// one workgroup, grid-stride: a diagnostic, not a step kernel
+__global__ void __launch_bounds__(1024) k_minmax(int n, const double *__restrict__ x, double *__restrict__ out) {
+    double mn = 1.0e16, mx = -1.0e16;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const double v = x[i];
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a = __shfl_down(mn, o, 64), b = __shfl_down(mx, o, 64);
+        if (a < mn) mn = a;
+        if (b > mx) mx = b;
+    }
+    __shared__ double smn[16], smx[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        smn[w] = mn;
+        smx[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 16; k++) {
+            if (smn[k] < mn) mn = smn[k];
+            if (smx[k] > mx) mx = smx[k];
+        }
+        out[0] = mn;
+        out[1] = mx;
+    }
+}
+
 }  // namespace hfh

@@ -22,6 +22,9 @@
   Loading locates each record by its point, as the reference does, so the
   record order does not matter.
 
+* ``print_vtk(sim, nameprint, ...)`` writes the 2D VTK file of
+  higflow_print_vtk2D (hig-flow-io.c:233-1020).
+
 ``sim`` is anything with ``size/centers/get/set`` of the field ids of
 ``higflow_amd.native.Flow`` (the oracle binding has the same interface).
 """
@@ -164,3 +167,78 @@ def load_properties(sim, prefix, dim=2, viscoelastic=False, missing="error"):
         if hit != len(c):
             raise ValueError("%s: %d of %d elements found in the file" % (path, hit, len(c)))
         sim.set(field, comp, vals)
+
+
+def _at(sim, fields, domain, dim, field, comp, center, x):
+    """compute_value_at_point / compute_facet_value_at_point: the stencil of
+    point x seen from center, applied as dp_interpolate_from_stencil does
+    (pdomain.c:1026-1054: -rhs, then the weighted values in stencil order)."""
+    ids, w, rhs = sim.stencil(domain, dim, center, x)
+    v = fields[(field, comp)]
+    res = 0.0
+    res += -rhs
+    for i, wi in zip(ids, w):
+        res += v[i] * wi
+    return res
+
+
+def print_vtk(sim, nameprint, rank=0, frame=0, Re=1.0, beta=0.0, viscoelastic=False, fields=None):
+    """Writes ``<nameprint>_<rank>-<frame>.vtk`` as higflow_print_vtk2D does
+    (hig-flow-io.c:233-1020, Newtonian and viscoelastic 2D flows):
+    * the leaf cells as VTK polygons, four corner points each;
+    * ``vel``: the velocity interpolated at each corner from the cell's
+      centre through the velocity-facet stencils;
+    * viscoelastic: ``τₚ`` at each corner, S through the tensor-cell stencil
+      of the corner plus (1 - beta) Du / Re, Du interpolated from the centre;
+    * ``p`` at the cell centres.
+
+    ``fields`` maps (field id, component) to values already on the host;
+    missing ones are read from ``sim``.  The stencils come from the
+    library's host mesh (``hfh_stencil``).  Returns the file name."""
+    if sim.dim != 2:
+        raise NotImplementedError("print_vtk: the 2D writer only (higflow_print_vtk2D)")
+    dim = 2
+    need = [(0, d) for d in range(dim)] + [(1, 0)]
+    if viscoelastic:
+        need += [(7, m) for m in range(dim * dim)] + [(8, m) for m in range(dim * dim)]
+    f = dict(fields or {})
+    for key in need:
+        if key not in f:
+            f[key] = sim.get(*key)
+    lo, hi = sim.cell_bounds()
+    cen = np.asarray(sim.centers(1, 0)).reshape(-1, dim)
+    n = len(cen)
+    corners = lambda k: ((lo[k, 0], lo[k, 1]), (hi[k, 0], lo[k, 1]), (hi[k, 0], hi[k, 1]), (lo[k, 0], hi[k, 1]))
+    name = "%s_%d-%d.vtk" % (nameprint, rank, frame)
+    out = ["# vtk DataFile Version 3.0\nhigtree\nASCII\nDATASET UNSTRUCTURED_GRID\n", "\nPOINTS %d float\n" % (4 * n)]
+    for k in range(n):
+        out.append("".join("%e %e 0\n" % p for p in corners(k)))
+    out.append("\nCELLS %d %d\n" % (n, 5 * n))
+    out.extend("%d %d %d %d %d\n" % (4, 4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3) for i in range(n))
+    out.append("\nCELL_TYPES %d\n" % n)
+    out.append("7 " * n)
+    out.append("\n\nPOINT_DATA %d\nVECTORS vel FLOAT\n" % (4 * n))
+    for k in range(n):
+        c = cen[k]
+        for p in corners(k):
+            x = np.array(p)
+            out.append("%e %e 0\n" % tuple(_at(sim, f, 1, d, 0, d, c, x) for d in range(dim)))
+    if viscoelastic:
+        out.append("\nTENSORS τₚ FLOAT\n")
+        for k in range(n):
+            c = cen[k]
+            for p in corners(k):
+                x = np.array(p)
+                t = [[0.0] * dim for _ in range(dim)]
+                for i in range(dim):
+                    for j in range(dim):
+                        Dp = _at(sim, f, 2, 0, 8, i * dim + j, c, x)
+                        t[i][j] = _at(sim, f, 2, 0, 7, i * dim + j, x, x)
+                        t[i][j] += 2.0 * (1 - beta) * 0.5 * (Dp) / Re
+                out.append("%e %e 0\n%e %e 0\n0 0 0\n" % (t[0][0], t[0][1], t[1][0], t[1][1]))
+    out.append("\nCELL_DATA %d\nSCALARS p FLOAT\nLOOKUP_TABLE default\n" % n)
+    for k in range(n):
+        out.append("%e\n" % _at(sim, f, 0, 0, 1, 0, cen[k], cen[k]))
+    with open(name, "w", encoding="utf-8") as fh:
+        fh.write("".join(out))
+    return name

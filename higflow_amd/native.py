@@ -82,6 +82,8 @@ def load():
     L.hfh_field_size.argtypes = [vp, C.c_int, C.c_int]
     for f in ("hfh_field_centers", "hfh_get_field", "hfh_set_field"):
         getattr(L, f).argtypes = [vp, C.c_int, C.c_int, dp]
+    L.hfh_field_range.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    L.hfh_cell_bounds.argtypes = [vp, dp]
     L.hfh_solver_step.argtypes = [vp]
     L.hfh_solver_step_viscoelastic.argtypes = [vp]
     L.hfh_set_time.argtypes = [vp, C.c_double]
@@ -202,6 +204,21 @@ class Flow:
         if v.size != self.size(field, comp):
             raise ValueError("field size mismatch")
         _check(self.L, self.L.hfh_set_field(self.h, field, comp, _dp(v)), "hfh_set_field")
+
+    def range(self, field, comp=0):
+        """(min, max) of a field, reduced on the device (for velocity: the
+        Vmin/Vmax the reference's final-velocity step prints)."""
+        mn, mx = C.c_double(0), C.c_double(0)
+        _check(self.L, self.L.hfh_field_range(self.h, field, comp, C.byref(mn), C.byref(mx)), "hfh_field_range")
+        return mn.value, mx.value
+
+    def cell_bounds(self):
+        """(lowpoints, highpoints) of the pressure cells, each (Nc, dim)."""
+        n = self.size(1)
+        out = np.zeros(2 * self.dim * n)
+        _check(self.L, self.L.hfh_cell_bounds(self.h, _dp(out)), "hfh_cell_bounds")
+        out = out.reshape(n, 2, self.dim)
+        return out[:, 0, :].copy(), out[:, 1, :].copy()
 
     def step(self):
         if self.ve:

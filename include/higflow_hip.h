@@ -127,6 +127,14 @@ int hfh_field_size(hfh_solver *s, int field, int comp);
 int hfh_field_centers(hfh_solver *s, int field, int comp, double *xyz);
 int hfh_get_field(hfh_solver *s, int field, int comp, double *out);
 int hfh_set_field(hfh_solver *s, int field, int comp, const double *in);
+/* min and max of a field on the device, from +-1e16 (for u[comp]: the
+ * Vmin/Vmax that higflow_final_velocity prints, hig-flow-step.c:250-306);
+ * returns the field size.  With HFH_VERBOSE=1 every step prints that line. */
+int hfh_field_range(hfh_solver *s, int field, int comp, double *vmin, double *vmax);
+/* lowpoint[DIM] then highpoint[DIM] of every pressure cell (hig_cell,
+ * higtree.h:59-75), in local-id order: the geometry the reference's VTK
+ * writer prints (hig-flow-io.c:233-265); returns the cell count */
+int hfh_cell_bounds(hfh_solver *s, double *lohi);
 
 /* higflow_solver_step (hig-flow-step.c:1363-1413); then t += dt */
 int hfh_solver_step(hfh_solver *s);

@@ -55,6 +55,27 @@ def test_cavity_small_parity():
     print(w)
 
 
+def test_velocity_range_log(monkeypatch, capfd):
+    # the Vmin/Vmax line of the reference's final-velocity step
+    # (hig-flow-step.c:302-306): a device reduction over the unknown facets
+    monkeypatch.setenv("HFH_VERBOSE", "1")
+    case = cases.cavity(16, **TIGHT)
+    o, h = hfo.Oracle(case), native.Flow(case)
+    for _ in range(3):
+        o.step()
+        h.step()
+    lines = [ln for ln in capfd.readouterr().out.splitlines() if ln.startswith("===>")]
+    assert len(lines) == 3 * 2
+    for d in range(2):
+        u = h.get(0, d)
+        assert h.range(0, d) == (u.min(), u.max())
+        uo = o.get(0, d)
+        scale = max(np.abs(uo).max(), 1e-300)
+        mn, mx = h.range(0, d)
+        assert abs(mn - uo.min()) <= 1e-10 * scale and abs(mx - uo.max()) <= 1e-10 * scale
+        assert lines[-2 + d] == "===> %d: Vmin = %15.10f <===> Vmax = %15.10f <===" % (d, mn, mx)
+
+
 def test_cavity_32_parity_20_steps():
     compare(cases.cavity(32, **TIGHT), 20)
 

@@ -74,3 +74,60 @@ def test_load_finds_records_by_point(tmp_path):
     b = hfo.Oracle(case)
     io.load_properties(b, prefix)
     assert np.array_equal(a.get(0, 0), b.get(0, 0))
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "newt_poiseuille.npz")
+
+
+def _vtk_section(lines, head, count):
+    i = lines.index(head)
+    return np.array([[float(v) for v in ln.split()] for ln in lines[i + 1:i + 1 + count]])
+
+
+def test_vtk_writer_on_the_channel_fixture(tmp_path, monkeypatch):
+    # higflow_print_vtk2D layout on the reference-held channel state: the
+    # corner velocities interpolate the Poiseuille parabola (quadratic WLS
+    # reproduces it; the fixture is steady to ~1e-7 and %e keeps 7 digits),
+    # the pressure is the cell value
+    monkeypatch.setenv("HFH_HOST_ONLY", "1")
+    from higflow_amd import native
+    fx = np.load(GOLD)
+    h = native.Flow(cases.newt_channel())
+    f = {(0, 0): fx["u0"][:, 2], (0, 1): fx["u1"][:, 2], (1, 0): fx["p"][:, 2]}
+    name = io.print_vtk(h, str(tmp_path / "newt"), rank=0, frame=3, fields=f)
+    assert name.endswith("newt_0-3.vtk")
+    lines = open(name, encoding="utf-8").read().split("\n")
+    n = 6400
+    assert lines[:4] == ["# vtk DataFile Version 3.0", "higtree", "ASCII", "DATASET UNSTRUCTURED_GRID"]
+    pts = _vtk_section(lines, "POINTS %d float" % (4 * n), 4 * n)
+    cells = _vtk_section(lines, "CELLS %d %d" % (n, 5 * n), n)
+    assert np.array_equal(cells[:, 0], np.full(n, 4)) and np.array_equal(cells[:, 1:].ravel(), np.arange(4 * n))
+    assert lines[lines.index("CELL_TYPES %d" % n) + 1] == "7 " * n
+    lo, hi = h.cell_bounds()
+    assert np.allclose(0.5 * (lo + hi), h.centers(1), rtol=0, atol=1e-15)
+    assert np.allclose(pts[0::4, :2], lo, rtol=1e-6, atol=1e-12) and np.allclose(pts[2::4, :2], hi, rtol=1e-6)
+    vel = _vtk_section(lines, "VECTORS vel FLOAT", 4 * n)
+    inner = (pts[:, 0] > 0.1) & (pts[:, 0] < 7.9)
+    assert np.abs(vel[inner, 0] - 1.5 * (1 - pts[inner, 1] ** 2)).max() < 1e-5
+    assert np.abs(vel[inner, 1]).max() < 1e-5
+    p = _vtk_section(lines, "LOOKUP_TABLE default", n)[:, 0]
+    assert np.array_equal(p, np.array(["%e" % v for v in fx["p"][:, 2]], dtype=float))
+
+
+def test_vtk_writer_viscoelastic_tensor(tmp_path, monkeypatch):
+    # constant S and Du: the tensor-cell stencils reproduce constants, so
+    # every corner prints tau_p = S + 2 (1 - beta) 0.5 Du / Re
+    monkeypatch.setenv("HFH_HOST_ONLY", "1")
+    from higflow_amd import native
+    h = native.Flow(cases.ve_cavity(8, Re=2.0, Wi=1.0, beta=0.25, dt=1e-4))
+    n = h.size(1)
+    S0, D0 = np.array([[1.5, -0.5], [0.25, 2.0]]), np.array([[0.1, 0.3], [-0.2, -0.1]])
+    f = {(0, 0): np.zeros(h.size(0, 0)), (0, 1): np.zeros(h.size(0, 1)), (1, 0): np.zeros(n)}
+    for m in range(4):
+        f[(7, m)] = np.full(h.size(7, m), S0.flat[m])
+        f[(8, m)] = np.full(h.size(8, m), D0.flat[m])
+    name = io.print_vtk(h, str(tmp_path / "ve"), Re=2.0, beta=0.25, viscoelastic=True, fields=f)
+    lines = open(name, encoding="utf-8").read().split("\n")
+    t = _vtk_section(lines, "TENSORS τₚ FLOAT", 12 * n).reshape(4 * n, 3, 3)
+    want = S0 + 2.0 * (1 - 0.25) * 0.5 * D0 / 2.0
+    assert np.abs(t[:, :2, :2] - want).max() < 1e-6 and not t[:, 2, :].any() and not t[:, :, 2].any()
