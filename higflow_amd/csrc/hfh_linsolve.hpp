@@ -398,6 +398,7 @@ class DeviceSystem {
     int max_restarts = 1;      // fused64: restarts on the true residual
     double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
     int n32_ = 0, n64_ = 0;              // timed solves of each form
+    hipEvent_t tev_[2] = {nullptr, nullptr};
     // the slower form is re-timed every probe_every_ solves; the interval
     // doubles (up to 1600) while the faster form keeps winning
     int since_probe_ = 0, probe_every_ = 50;
@@ -420,6 +421,8 @@ class DeviceSystem {
     DeviceSystem &operator=(const DeviceSystem &) = delete;
     ~DeviceSystem() {
         if (hpin_) (void)hipHostFree(hpin_);
+        for (auto &e : tev_)
+            if (e) (void)hipEventDestroy(e);
     }
 
     // `amg`: precondition the fused BiCGStab with a smoothed-aggregation
@@ -607,8 +610,8 @@ class DeviceSystem {
     }
 
     // Mixed form: per system, FP32 refinement or the FP64 solve, whichever
-    // took less wall time on its last solves (both are timed on the host
-    // clock; each solve ends with a host synchronisation).  The first four
+    // took less time on its last solves (timed with events on the stream;
+    // each solve ends with a host synchronisation).  The first six
     // solves alternate the forms; after that the form not in use is re-timed
     // every 50 solves.  Block-diagonal systems refine all segments together,
     // each against its own tolerance.
@@ -618,7 +621,7 @@ class DeviceSystem {
         const bool pref32 = t_mixed_ >= 0 && t_fp64_ >= 0 && t_mixed_ < t_fp64_;
         if (force32_)
             use32 = true;
-        else if ((n64_ < 2 || n32_ < 2) && t_mixed_ < 1e30)
+        else if ((n64_ < 3 || n32_ < 3) && t_mixed_ < 1e30)
             use32 = n32_ < n64_;  // first solves: the forms alternate, FP64 first
         else {
             use32 = pref32;
@@ -628,7 +631,14 @@ class DeviceSystem {
                 probing = true;
             }
         }
-        const auto t0 = std::chrono::steady_clock::now();
+        // each form is timed on the stream between two events: the solve's
+        // launches and the idle gaps of its host checks, not the tail of
+        // work queued before it
+        if (!tev_[0]) {
+            HFH_CHECK(hipEventCreate(&tev_[0]));
+            HFH_CHECK(hipEventCreate(&tev_[1]));
+        }
+        HFH_CHECK(hipEventRecord(tev_[0], st));
         SolveStats S;
         if (use32) {
             S = mixed_refine(spd, b, x, rtol, atol, inner_rtol, maxit, st);
@@ -646,9 +656,14 @@ class DeviceSystem {
             S.fallback = 2;
             for (auto &g : seg_stats) g.fallback = 2;
         }
-        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        HFH_CHECK(hipEventRecord(tev_[1], st));
+        HFH_CHECK(hipEventSynchronize(tev_[1]));
+        float ms = 0;
+        HFH_CHECK(hipEventElapsedTime(&ms, tev_[0], tev_[1]));
+        const double dt = 1e-3 * ms;
         // the second solve of a form replaces the time of its first (which
-        // pays one-time costs: first launches, no iteration prediction yet)
+        // pays one-time costs: first launches, no iteration prediction yet);
+        // later solves enter a running mean
         if (use32 && S.fallback == 0)
             t_mixed_ = n32_++ < 2 ? dt : 0.5 * (t_mixed_ + dt);
         else if (use32)
