@@ -238,7 +238,14 @@ def main():
                              {"fp64_max_iters": [max(sim64.solve_stats(m, w)["iters"] for m in range(sim64.n))
                                                  for w in range(D + 1)],
                               "fp64_all_converged": all(sim64.solve_stats(m, w)["converged"] == 1
-                                                        for m in range(sim64.n) for w in range(D + 1))}),
+                                                        for m in range(sim64.n) for w in range(D + 1)),
+                              # per system: the mixed form's last solve (0 FP32 refinement, 1 FP32
+                              # failed and finished in FP64, 2 FP64 chosen), its iterations and
+                              # corrections (the same for every member of a block-diagonal solve)
+                              "mixed_form": [{k3: simmx.solve_stats(0, w)[k3] for k3 in ("fallback", "iters", "outer")}
+                                             for w in range(D + 1)],
+                              "mixed_all_converged": all(simmx.solve_stats(m, w)["converged"] == 1
+                                                         for m in range(simmx.n) for w in range(D + 1))}),
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -15,7 +15,8 @@
 // V(1,1) cycle with damped Jacobi (omega 0.6):
 //   x = omega D^-1 b; r = b - A x; b_c = R r; x_c = V(b_c); x += P x_c;
 //   x += omega D^-1 (b - A x)
-// Each kernel launch handles one such step for one level; rows run on L
+// Each kernel launch handles one such step for one level, except the last
+// two, which are one launch through A P (k_amg_prolong_post); rows run on L
 // lanes as in the Krylov kernels.  Every launch first reads the solve's done
 // flag and returns when a previous launch has set it.
 #pragma once
@@ -23,6 +24,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <vector>
 
@@ -175,7 +178,7 @@ inline int sa_aggregate(const HCsr &A, double theta, std::vector<int> &agg) {
 }
 
 struct AmgHostLevel {
-    HCsr A, P, R;
+    HCsr A, P, R, AP;  // AP = A P (the fused prolongation + post-smoothing)
     std::vector<double> dinv;
 };
 
@@ -228,8 +231,8 @@ inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<
             for (int k = A.rp[i]; k < A.rp[i + 1]; k++) S.v[k] = -w * L.dinv[i] * A.v[k] + (A.ci[k] == i ? 1.0 : 0.0);
         L.P = csr_mul(S, Pt);
         L.R = csr_transpose(L.P);
-        HCsr AP = csr_mul(A, L.P);
-        A = csr_mul(L.R, AP);
+        L.AP = csr_mul(A, L.P);
+        A = csr_mul(L.R, L.AP);
         lv.push_back(std::move(L));
     }
     if (lv.empty() || A.nrow > 2000) return false;  // coarsening stalled
@@ -388,6 +391,29 @@ __global__ void __launch_bounds__(ANT) k_amg_post(int n, const int *__restrict__
     if (in && threadIdx.x % AL == 0) y[i] = x[i] + omega * (dinv[i] * (b[i] - s));
 }
 
+// y = omega D^-1 b + P x_c + omega D^-1 (r - (A P) x_c): prolongation and
+// post-smoothing of one level in one launch.  r = b - A (omega D^-1 b) from
+// the level's residual launch, so for x = omega D^-1 b + P x_c the smoother's
+// b - A x is r - (A P) x_c, with A P stored at set-up.
+template <typename T>
+__global__ void __launch_bounds__(ANT) k_amg_prolong_post(int n, const int *__restrict__ prp,
+                                                          const int *__restrict__ pci, const T *__restrict__ pv,
+                                                          const int *__restrict__ qrp, const int *__restrict__ qci,
+                                                          const T *__restrict__ qv, const T *__restrict__ dinv,
+                                                          T omega, const T *__restrict__ b, const T *__restrict__ r,
+                                                          const T *__restrict__ xc, T *__restrict__ y,
+                                                          const int *ki) {
+    if (amg_done(ki)) return;
+    const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
+    const bool in = i < n;
+    const T s1 = arow<T>(in ? prp[i] : 0, in ? prp[i + 1] : 0, pci, pv, xc);
+    const T s2 = arow<T>(in ? qrp[i] : 0, in ? qrp[i + 1] : 0, qci, qv, xc);
+    if (in && threadIdx.x % AL == 0) {
+        const T di = dinv[i];
+        y[i] = (omega * (di * b[i]) + s1) + omega * (di * (r[i] - s2));
+    }
+}
+
 // The hierarchy on the device.  apply(b, y): y = V(b); b and y have the
 // fine size, y must differ from b.
 template <typename T>
@@ -396,6 +422,9 @@ class AmgPrecond {
     bool ready = false;
     int nlev = 0, ncoarse = 0;
     T omega = T(0.6);
+    // prolongation and post-smoothing fused through A P (HFH_AMG_FUSE=0:
+    // two launches per level, as x += P x_c, x += omega D^-1 (b - A x))
+    bool fuse_pp = true;
 
     bool build(int n, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<double> &v,
                hipStream_t st) {
@@ -411,6 +440,10 @@ class AmgPrecond {
         // 1000 rows (two launches) is cheaper than further levels (four
         // launches each)
         if (!sa_build(A, hl, cinv, ncoarse, 1000)) return false;
+        {
+            const char *e = std::getenv("HFH_AMG_FUSE");
+            fuse_pp = !(e && std::strcmp(e, "0") == 0);
+        }
         nlev = (int)hl.size();
         lv_.clear();
         lv_.resize(nlev);
@@ -428,6 +461,11 @@ class AmgPrecond {
             upi(d.Rrp, hl[l].R.rp, st);
             upi(d.Rci, hl[l].R.ci, st);
             upt(d.Rv, hl[l].R.v, st);
+            if (fuse_pp) {
+                upi(d.Qrp, hl[l].AP.rp, st);
+                upi(d.Qci, hl[l].AP.ci, st);
+                upt(d.Qv, hl[l].AP.v, st);
+            }
             d.r.alloc(d.n);
             d.x.alloc(d.n);
             d.bnext.alloc(d.nc);
@@ -444,15 +482,19 @@ class AmgPrecond {
 
     long bytes_per_cycle() const {
         long by = 0;
-        for (const Dev &d : lv_) by += 3L * (d.Arp.n * 4 + d.Av.n * (4 + sizeof(T))) + (d.Prp.n + d.Rrp.n) * 4;
+        for (const Dev &d : lv_) {
+            const int apass = fuse_pp ? 2 : 3;  // A rows read per level (the fused launch reads A P instead)
+            by += apass * (d.Arp.n * 4 + d.Av.n * (4 + sizeof(T))) + (d.Prp.n + d.Rrp.n) * 4;
+            if (fuse_pp) by += d.Qrp.n * 4 + d.Qv.n * (4 + sizeof(T));
+        }
         return by;
     }
 
   private:
     struct Dev {
         int n = 0, nc = 0;
-        KBuf<int> Arp, Aci, Prp, Pci, Rrp, Rci;
-        KBuf<T> Av, dinv, Pv, Rv, r, x, bnext, xnext;
+        KBuf<int> Arp, Aci, Prp, Pci, Rrp, Rci, Qrp, Qci;  // Q = A P
+        KBuf<T> Av, dinv, Pv, Rv, Qv, r, x, bnext, xnext;
     };
     std::vector<Dev> lv_;
     KBuf<T> cinv_;
@@ -486,6 +528,11 @@ class AmgPrecond {
             k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
             k_amg_dense<T><<<std::max(1, (ncoarse * 64 + ANT - 1) / ANT), ANT, 0, st>>>(ncoarse, cinv_.p, d.bnext.p,
                                                                                         d.xnext.p, ki);
+        }
+        if (fuse_pp) {
+            k_amg_prolong_post<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p, d.Qv.p,
+                                                             d.dinv.p, omega, b, d.r.p, d.xnext.p, y, ki);
+            return;
         }
         k_amg_prolong<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b, d.xnext.p,
                                                     d.x.p, ki);

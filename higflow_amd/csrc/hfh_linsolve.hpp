@@ -394,14 +394,14 @@ class DeviceSystem {
     FusedKrylov<double> k64;
     FusedKrylov<float> k32;
     bool fused = true;
-    int fp64_solves_left = 0;  // mixed: solves to run in FP64 after an FP32 failure
     int max_restarts = 1;      // fused64: restarts on the true residual
     double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
     int n32_ = 0, n64_ = 0;              // timed solves of each form
     hipEvent_t tev_[2] = {nullptr, nullptr};
     // the slower form is re-timed every probe_every_ solves; the interval
-    // doubles (up to 1600) while the faster form keeps winning
-    int since_probe_ = 0, probe_every_ = 50;
+    // doubles (up to 1600) while the faster form keeps winning, and is 10
+    // again after a flip
+    int since_probe_ = 0, probe_every_ = 10;
     int nseg = 1;
     std::vector<int> seg_off;
     std::vector<SolveStats> seg_stats;  // last solve, per segment
@@ -488,7 +488,7 @@ class DeviceSystem {
         t_mixed_ = t_fp64_ = -1;
         n32_ = n64_ = 0;
         since_probe_ = 0;
-        probe_every_ = 50;
+        probe_every_ = 10;
         {
             const char *em = std::getenv("HFH_MIXED_FP32");
             force32_ = em && std::strcmp(em, "1") == 0;
@@ -612,9 +612,13 @@ class DeviceSystem {
     // Mixed form: per system, FP32 refinement or the FP64 solve, whichever
     // took less time on its last solves (timed with events on the stream;
     // each solve ends with a host synchronisation).  The first six
-    // solves alternate the forms; after that the form not in use is re-timed
-    // every 50 solves.  Block-diagonal systems refine all segments together,
-    // each against its own tolerance.
+    // solves alternate the forms (the first steps of a run are not typical:
+    // the initial guesses improve as the solution history fills); after that
+    // the form not in use is re-timed after 10 solves, the interval doubling
+    // while the choice holds (up to 1600) and returning to 10 when it flips.
+    // A refinement whose FP32 solve fails finishes in FP64,
+    // and FP32 is tried again 50 solves later.  Block-diagonal systems refine
+    // all segments together, each against its own tolerance.
     SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                            int maxit, hipStream_t st) {
         bool use32, probing = false;
@@ -663,27 +667,26 @@ class DeviceSystem {
         const double dt = 1e-3 * ms;
         // the second solve of a form replaces the time of its first (which
         // pays one-time costs: first launches, no iteration prediction yet);
-        // later solves enter a running mean
+        // a re-timing replaces the form's older time; the form in use keeps
+        // a running mean
         if (use32 && S.fallback == 0)
-            t_mixed_ = n32_++ < 2 ? dt : 0.5 * (t_mixed_ + dt);
+            t_mixed_ = (n32_++ < 2 || t_mixed_ >= 1e30 || probing) ? dt : 0.5 * (t_mixed_ + dt);
         else if (use32)
             t_mixed_ = 1e30;  // FP32 failed on this system: keep FP64
         else
-            t_fp64_ = n64_++ < 2 ? dt : 0.5 * (t_fp64_ + dt);
-        if (probing) probe_every_ = (t_mixed_ < t_fp64_) == pref32 ? std::min(2 * probe_every_, 1600) : 50;
+            t_fp64_ = (n64_++ < 2 || probing) ? dt : 0.5 * (t_fp64_ + dt);
+        if (probing) probe_every_ = (t_mixed_ < t_fp64_) == pref32 ? std::min(2 * probe_every_, 1600) : 10;
+        if (use32 && S.fallback != 0 && !force32_) {
+            // FP32 failed: FP64 for the next 50 solves, then FP32 is tried again
+            since_probe_ = 0;
+            probe_every_ = 50;
+        }
         return S;
     }
 
     // FP64 defect correction around FP32 solves (see fused_mixed)
     SolveStats mixed_refine(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                             int maxit, hipStream_t st) {
-        if (fp64_solves_left > 0) {
-            // the FP32 corrections failed on this system recently
-            fp64_solves_left--;
-            SolveStats S = fused64(spd, b, x, rtol, atol, maxit, st);
-            S.fallback = 1;
-            return S;
-        }
         SolveStats S;
         double h[S_NSC];
         int hi[KI_N];
@@ -764,8 +767,7 @@ class DeviceSystem {
             if (!(hi[KI_DONE] && hi[KI_CONV])) {
                 // FP32 Krylov cannot reach inner_rtol on this system (too
                 // ill-conditioned for single precision): finish in FP64 from
-                // the current x and keep FP64 for the next solves
-                fp64_solves_left = 20;
+                // the current x (the caller keeps FP64 for a while)
                 SolveStats F = fused64(spd, b, x, rtol, atol, maxit, st);
                 F.iters += inner_total;
                 F.outer = outer;

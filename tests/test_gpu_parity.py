@@ -230,6 +230,22 @@ def test_fused_and_classic_krylov_agree(which, monkeypatch):
     _agree(hf, hc, case["dim"], case["params"]["flowtype"] == 3, 1e-10)
 
 
+def test_amg_fused_prolongation_agrees(monkeypatch):
+    # the V-cycle's prolongation and post-smoothing in one launch through
+    # A P against the two-launch form: the same converged step
+    case = cases.contraction(levels=2, h0=0.5, length=8.0, radius=1.0, Re=0.01, Wi=1.0, beta=1.0 / 9.0,
+                             dt=1e-4, rtol=1e-12, atol=1e-14, maxit=50000)
+    monkeypatch.setenv("HFH_AMG_FUSE", "0")
+    a = native.Flow(case)
+    monkeypatch.setenv("HFH_AMG_FUSE", "1")
+    b = native.Flow(case)
+    for _ in range(3):
+        a.step()
+        b.step()
+    _agree(b, a, 2, True, 1e-10)
+    assert b.solve_stats(0)["converged"] == 1
+
+
 @pytest.mark.parametrize("force32", ["0", "1"])
 def test_mixed_precision_amr_contraction(force32, monkeypatch):
     # nonsymmetric systems: FP32 BiCGStab corrections inside FP64 refinement,
