@@ -46,6 +46,9 @@ WORKLOADS = {
     "cavity2d_256_re100": dict(kind="cavity", n=256, Re=100.0, dt=1e-4, steps=100),
     # 2D Oldroyd-B cavity (log-conformation), uniform 128x128
     "ve_cavity2d_128": dict(kind="ve_cavity", n=128, Re=1.0, Wi=1.0, beta=0.5, dt=1e-5, steps=100),
+    # one large simulation: Oldroyd-B cavity, uniform 1024x1024 (1,048,576
+    # cells), Re=10, Wi=0.5, beta=0.5, semi-implicit Euler, dt=2e-4
+    "ve_cavity2d_1024": dict(kind="ve_cavity", n=1024, Re=10.0, Wi=0.5, beta=0.5, dt=2e-4, steps=50, semi=True),
     # batched sweep: 64 independent Oldroyd-B cavities from the seed-0
     # PCG32 table (sides 32..256, Re log-uniform in [1, 100], Wi in
     # [0.1, 2]; 1,225,216 cells), beta=0.5, dt=5e-4, semi-implicit Euler,
@@ -64,6 +67,8 @@ def make_case(spec, **tol):
     if spec["kind"] == "contraction":
         return cases.contraction(levels=spec["levels"], dt=spec["dt"], graded=spec.get("graded", False),
                                  h0=spec.get("h0", 0.1), **tol)
+    if spec.get("semi"):
+        tol = dict(tol, tempdiscrtype=cases.SEMI_IMPLICIT_EULER)
     return cases.ve_cavity(spec["n"], Re=spec["Re"], Wi=spec["Wi"], beta=spec["beta"], dt=spec["dt"], **tol)
 
 
