@@ -267,21 +267,39 @@ class FusedKrylov {
         (void)hipEventDestroy(b);
         return reps > 0 ? ms / reps : 0.0f;
     }
+    // one BiCGStab iteration's two launches (A and B) with L lanes per row,
+    // on zero vectors with the flags cleared and launch A's scalars set so
+    // that it runs its update and SpMV (as probe_a): reads and writes only
+    // buffers of this solver.  Best of two timings of 25 launches each.
     template <int L>
     float time_lanes(hipStream_t st) {
         lanes = L;
         nb = blocks<L>();
         part.alloc((size_t)KP_N * nb);
-        // launch B on zero vectors with the flags cleared: reads and writes
-        // only buffers of this solver
-        for (KBuf<T> *b : {&r, &rh, &v}) k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, b->p, T(0));
+        for (KBuf<T> *b : {&r, &rh, &v, &v2, &s, &t, &p, &p2, &q}) k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, b->p, T(0));
         k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
         k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
         k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
         time_b<L>(2, st);
-        return time_b<L>(20, st);
+        const float tb = std::min(time_b<L>(25, st), time_b<L>(25, st));
+        const T slotv[4] = {T(1), T(1), T(2), T(1)};  // t.s, t.t, rh.s, rh.t
+        const int slot[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
+        std::vector<T> hk(KS_N, T(0));
+        hk[KS_RHO + 0] = T(1);
+        hk[KS_ALPHA + 0] = T(0.5);
+        if (hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(ks.p, hk.data(), KS_N * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("lane timing set-up failed");
+        for (int k = 0; k < 4; k++)
+            if (hipMemcpy(part.p + (size_t)slot[k] * nb, &slotv[k], sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+                throw std::runtime_error("lane timing set-up failed");
+        time_a<L>(2, st);
+        const float ta = std::min(time_a<L>(25, st), time_a<L>(25, st));
+        k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
+        return ta + tb;
     }
-    // picks the lanes per row with the fastest SpMV launch.  The choice is
+    // picks the lanes per row with the fastest iteration (launches A and B
+    // of BiCGStab).  The choice is
     // kept per matrix shape (rows, entries, segments, precision) for the
     // life of the process, so solvers built later for the same system (a
     // second run of the same case) sum in the same order and give the same
