@@ -12,7 +12,7 @@
 //              lambda the spectral radius of D^-1 A by 15 power steps
 //   R = P^T, A_c = R A P, until at most `coarse_max` unknowns remain; the
 //   coarsest level is inverted densely (Gauss-Jordan, partial pivoting).
-// V(1,1) cycle with damped Jacobi (omega 0.6):
+// V(1,1) cycle with damped Jacobi (omega 0.8):
 //   x = omega D^-1 b; r = b - A x; b_c = R r; x_c = V(b_c); x += P x_c;
 //   x += omega D^-1 (b - A x)
 // Each kernel launch handles one such step for one level, except the last
@@ -421,7 +421,7 @@ class AmgPrecond {
   public:
     bool ready = false;
     int nlev = 0, ncoarse = 0;
-    T omega = T(0.6);
+    T omega = T(0.8);
     // prolongation and post-smoothing fused through A P (HFH_AMG_FUSE=0:
     // two launches per level, as x += P x_c, x += omega D^-1 (b - A x))
     bool fuse_pp = true;
@@ -439,7 +439,12 @@ class AmgPrecond {
         // stop coarsening at <= 1000 unknowns: one dense solve of up to
         // 1000 rows (two launches) is cheaper than further levels (four
         // launches each)
-        if (!sa_build(A, hl, cinv, ncoarse, 1000)) return false;
+        // HFH_AMG_THETA / HFH_AMG_OMEGA: strength threshold and Jacobi
+        // damping for comparison runs (defaults 0.08 and 0.8)
+        double theta = 0.08;
+        if (const char *e = std::getenv("HFH_AMG_THETA")) theta = std::atof(e);
+        if (const char *e = std::getenv("HFH_AMG_OMEGA")) omega = (T)std::atof(e);
+        if (!sa_build(A, hl, cinv, ncoarse, 1000, theta)) return false;
         {
             const char *e = std::getenv("HFH_AMG_FUSE");
             fuse_pp = !(e && std::strcmp(e, "0") == 0);
