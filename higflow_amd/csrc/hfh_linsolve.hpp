@@ -397,6 +397,10 @@ class DeviceSystem {
     int max_restarts = 1;      // fused64: restarts on the true residual
     double t_mixed_ = -1, t_fp64_ = -1;  // mixed form: seconds per solve of each form
     int n32_ = 0, n64_ = 0;              // timed solves of each form
+    bool explored_ = false;              // the first six solves are done
+    struct Pending {                     // a solve whose time is not booked yet
+        bool on = false, use32 = false, ok = false, probing = false, pref32 = false;
+    } pend_;
     hipEvent_t tev_[2] = {nullptr, nullptr};
     // the slower form is re-timed every probe_every_ solves; the interval
     // doubles (up to 1600) while the faster form keeps winning, and is 10
@@ -487,6 +491,8 @@ class DeviceSystem {
         }
         t_mixed_ = t_fp64_ = -1;
         n32_ = n64_ = 0;
+        explored_ = false;
+        pend_ = Pending();
         since_probe_ = 0;
         probe_every_ = 10;
         {
@@ -614,13 +620,15 @@ class DeviceSystem {
     // each solve ends with a host synchronisation).  The first six
     // solves alternate the forms (the first steps of a run are not typical:
     // the initial guesses improve as the solution history fills); after that
-    // the form not in use is re-timed after 10 solves, the interval doubling
-    // while the choice holds (up to 1600) and returning to 10 when it flips.
+    // the form not in use is re-timed after 10 solves (50 when it was 30%
+    // slower), the interval doubling (quadrupling when 30% slower) while the
+    // choice holds, up to 1600, and returning to 10 when it flips.
     // A refinement whose FP32 solve fails finishes in FP64,
     // and FP32 is tried again 50 solves later.  Block-diagonal systems refine
     // all segments together, each against its own tolerance.
     SolveStats fused_mixed(bool spd, const double *b, double *x, double rtol, double atol, double inner_rtol,
                            int maxit, hipStream_t st) {
+        account_pending();
         bool use32, probing = false;
         const bool pref32 = t_mixed_ >= 0 && t_fp64_ >= 0 && t_mixed_ < t_fp64_;
         if (force32_)
@@ -637,7 +645,9 @@ class DeviceSystem {
         }
         // each form is timed on the stream between two events: the solve's
         // launches and the idle gaps of its host checks, not the tail of
-        // work queued before it
+        // work queued before it.  The time is read at the next solve of this
+        // system, when the end event has long completed, so no extra host
+        // wait is added.
         if (!tev_[0]) {
             HFH_CHECK(hipEventCreate(&tev_[0]));
             HFH_CHECK(hipEventCreate(&tev_[1]));
@@ -648,7 +658,7 @@ class DeviceSystem {
             S = mixed_refine(spd, b, x, rtol, atol, inner_rtol, maxit, st);
             if (!S.converged && S.fallback == 0) {
                 // the refinement stalled (FP32 too coarse for this system):
-                // finish in FP64 from the current x, keep FP64 from now on
+                // finish in FP64 from the current x
                 SolveStats F = fused64(spd, b, x, rtol, atol, maxit, st);
                 F.iters += S.iters;
                 F.outer = S.outer;
@@ -661,27 +671,44 @@ class DeviceSystem {
             for (auto &g : seg_stats) g.fallback = 2;
         }
         HFH_CHECK(hipEventRecord(tev_[1], st));
+        pend_ = Pending{true, use32, S.fallback == 0, probing, pref32};
+        return S;
+    }
+
+    // books the time of the last mixed-form solve (see fused_mixed)
+    void account_pending() {
+        if (!pend_.on) return;
+        pend_.on = false;
         HFH_CHECK(hipEventSynchronize(tev_[1]));
         float ms = 0;
         HFH_CHECK(hipEventElapsedTime(&ms, tev_[0], tev_[1]));
         const double dt = 1e-3 * ms;
+        const bool use32 = pend_.use32, probing = pend_.probing, pref32 = pend_.pref32;
         // the second solve of a form replaces the time of its first (which
         // pays one-time costs: first launches, no iteration prediction yet);
         // a re-timing replaces the form's older time; the form in use keeps
         // a running mean
-        if (use32 && S.fallback == 0)
+        if (use32 && pend_.ok)
             t_mixed_ = (n32_++ < 2 || t_mixed_ >= 1e30 || probing) ? dt : 0.5 * (t_mixed_ + dt);
         else if (use32)
             t_mixed_ = 1e30;  // FP32 failed on this system: keep FP64
         else
             t_fp64_ = (n64_++ < 2 || probing) ? dt : 0.5 * (t_fp64_ + dt);
-        if (probing) probe_every_ = (t_mixed_ < t_fp64_) == pref32 ? std::min(2 * probe_every_, 1600) : 10;
-        if (use32 && S.fallback != 0 && !force32_) {
+        // a clearly slower form (30% or more) is re-timed less often
+        const double ratio = std::max(t_mixed_, t_fp64_) / std::max(1e-30, std::min(t_mixed_, t_fp64_));
+        if (probing)
+            probe_every_ =
+                (t_mixed_ < t_fp64_) == pref32 ? std::min((ratio > 1.3 ? 4 : 2) * probe_every_, 1600) : 10;
+        if (!explored_ && n32_ >= 3 && n64_ >= 3) {
+            explored_ = true;
+            since_probe_ = 0;
+            probe_every_ = ratio > 1.3 ? 50 : 10;
+        }
+        if (use32 && !pend_.ok && !force32_) {
             // FP32 failed: FP64 for the next 50 solves, then FP32 is tried again
             since_probe_ = 0;
             probe_every_ = 50;
         }
-        return S;
     }
 
     // FP64 defect correction around FP32 solves (see fused_mixed)
