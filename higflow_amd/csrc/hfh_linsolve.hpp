@@ -346,6 +346,47 @@ __global__ void __launch_bounds__(BLOCK) k_residual(int n, const int *__restrict
     }
 }
 
+// the same with 4 lanes per row: workgroups of 4 BLOCK threads, one per
+// BLOCK rows, so the partials have k_residual's layout (one per BLOCK rows)
+template <typename T>
+__global__ void __launch_bounds__(4 * BLOCK) k_residual_l4(int n, const int *__restrict__ rp,
+                                                           const int *__restrict__ ci, const T *__restrict__ v,
+                                                           const T *__restrict__ x, const T *__restrict__ b,
+                                                           T *__restrict__ r, T *__restrict__ prr,
+                                                           T *__restrict__ pbb) {
+    const int i = blockIdx.x * BLOCK + (int)threadIdx.x / 4, lane = threadIdx.x & 3;
+    T s = 0;
+    if (i < n)
+        for (int k = rp[i] + lane; k < rp[i + 1]; k += 4) s += v[k] * x[ci[k]];
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    T rr = 0, bb = 0;
+    if (i < n && lane == 0) {
+        const T bi = b[i], ri = bi - s;
+        r[i] = ri;
+        rr = ri * ri;
+        bb = bi * bi;
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    __shared__ T srr[4 * BLOCK / 64], sbb[4 * BLOCK / 64];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        srr[w] = rr;
+        sbb[w] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T a = 0, c = 0;
+        for (int q = 0; q < 4 * BLOCK / 64; q++) {
+            a += srr[q];
+            c += sbb[q];
+        }
+        prr[blockIdx.x] = a;
+        pbb[blockIdx.x] = c;
+    }
+}
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -576,7 +617,7 @@ class DeviceSystem {
             k64.init<double>(spd, b, x, st);
             S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
             // true residual per segment (segments start on 256-row blocks)
-            k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
             HFH_CHECK(hipMemcpyAsync(rr.data(), part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipMemcpyAsync(bb.data(), part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipStreamSynchronize(st));
@@ -726,7 +767,7 @@ class DeviceSystem {
         std::vector<double> prr(nseg > 1 ? nb : 0), pbb(nseg > 1 ? nb : 0);
         std::vector<SolveStats> gs(nseg);
         for (;;) {
-            k_residual<double><<<nb, BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
             if (nseg == 1) {
                 k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
                 HFH_CHECK(hipMemcpyAsync(hpin_ + off_sc_, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
