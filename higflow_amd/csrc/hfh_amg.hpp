@@ -351,16 +351,16 @@ __global__ void __launch_bounds__(1024) k_amg_coarse(int nc, const int *__restri
 
 // coarsest level, many workgroups: x_c = C^-1 b_c with b_c already
 // restricted (a wave per row)
-template <typename T>
-__global__ void __launch_bounds__(ANT) k_amg_dense(int nc, const T *__restrict__ cinv, const T *__restrict__ bc,
+template <typename T, typename TC>
+__global__ void __launch_bounds__(ANT) k_amg_dense(int nc, const TC *__restrict__ cinv, const T *__restrict__ bc,
                                                    T *__restrict__ xc, const int *ki) {
     if (amg_done(ki)) return;
     const int lane = threadIdx.x & 63;
     const int i = (blockIdx.x * ANT + threadIdx.x) >> 6;
     if (i >= nc) return;
-    const T *row = cinv + (size_t)i * nc;
+    const TC *row = cinv + (size_t)i * nc;
     T s = 0;
-    for (int j = lane; j < nc; j += 64) s += row[j] * bc[j];
+    for (int j = lane; j < nc; j += 64) s += (T)row[j] * bc[j];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) xc[i] = s;
 }
@@ -477,6 +477,15 @@ class AmgPrecond {
             d.xnext.alloc(d.nc);
         }
         upt(cinv_, cinv, st);
+        {
+            std::vector<float> c32(cinv.begin(), cinv.end());
+            cinv32_.alloc(c32.size());
+            if (!c32.empty() &&
+                (hipMemcpyAsync(cinv32_.p, c32.data(), c32.size() * sizeof(float), hipMemcpyHostToDevice, st) !=
+                     hipSuccess ||
+                 hipStreamSynchronize(st) != hipSuccess))
+                throw std::runtime_error("amg upload failed");
+        }
         if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("amg upload failed");
         ready = true;
         return true;
@@ -503,6 +512,7 @@ class AmgPrecond {
     };
     std::vector<Dev> lv_;
     KBuf<T> cinv_;
+    KBuf<float> cinv32_;  // the dense coarsest inverse as read by k_amg_dense
 
     static void upi(KBuf<int> &d, const std::vector<int> &h, hipStream_t st) {
         d.alloc(h.size());
@@ -531,8 +541,11 @@ class AmgPrecond {
         } else {
             // larger coarsest level: restrict, then a wave per dense row
             k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
-            k_amg_dense<T><<<std::max(1, (ncoarse * 64 + ANT - 1) / ANT), ANT, 0, st>>>(ncoarse, cinv_.p, d.bnext.p,
-                                                                                        d.xnext.p, ki);
+            // the dense inverse is read in FP32 (half the bytes of the
+            // coarsest solve; it is part of a preconditioner, and the Krylov
+            // iteration around it keeps its precision)
+            k_amg_dense<T, float><<<std::max(1, (ncoarse * 64 + ANT - 1) / ANT), ANT, 0, st>>>(
+                ncoarse, cinv32_.p, d.bnext.p, d.xnext.p, ki);
         }
         if (fuse_pp) {
             k_amg_prolong_post<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p, d.Qv.p,
