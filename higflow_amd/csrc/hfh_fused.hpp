@@ -100,13 +100,13 @@ class FusedKrylov {
     // Jacobi-PCG (symmetric positive definite systems); with a V-cycle the
     // solve is AMG-BiCGStab in either case.
     template <typename TB>
-    void init(bool cg, const TB *b, T *x, hipStream_t st) {
+    void init(bool cg, const TB *b, T *x, hipStream_t st, bool xzero = false) {
         cg = cg && !amg.ready;
         switch (lanes) {
-        case 1: launch_init<TB, 1>(cg, b, x, st); break;
-        case 2: launch_init<TB, 2>(cg, b, x, st); break;
-        case 4: launch_init<TB, 4>(cg, b, x, st); break;
-        default: launch_init<TB, 8>(cg, b, x, st); break;
+        case 1: launch_init<TB, 1>(cg, xzero, b, x, st); break;
+        case 2: launch_init<TB, 2>(cg, xzero, b, x, st); break;
+        case 4: launch_init<TB, 4>(cg, xzero, b, x, st); break;
+        default: launch_init<TB, 8>(cg, xzero, b, x, st); break;
         }
     }
     // iterations [it0, it0 + count)
@@ -212,8 +212,9 @@ class FusedKrylov {
     }
 
     template <typename TB, int L>
-    void launch_init(bool cg, const TB *b, T *x, hipStream_t st) {
-        k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, rp, ci, val.p, x, b, dinv.p, r.p, z.p, rh.p, part.p, nb,
+    void launch_init(bool cg, bool xzero, const TB *b, T *x, hipStream_t st) {
+        k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, xzero ? 1 : 0, rp, ci, val.p, x, b, dinv.p, r.p, z.p,
+                                              rh.p, part.p, nb,
                                               ki.p, S);
     }
     template <int L>

@@ -242,21 +242,26 @@ __device__ inline void row_sums(int rb, int re, const int *__restrict__ ci, cons
 
 // ------------------------------------------------------------------ init
 // r = b - A x; CG: z = dinv r, partial r.z; BiCGStab: rh = r, partial r.r.
-// Partials r.r and b.b; clears the flags.
+// Partials r.r and b.b; clears the flags.  xzero: x is set to zero here and
+// r = b (no product; the corrections of a refinement start from zero).
 template <typename T, typename TB, int L>
-__global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, const int *__restrict__ rp, const int *__restrict__ ci,
-                                               const T *__restrict__ val, const T *__restrict__ x,
-                                               const TB *__restrict__ b, const T *__restrict__ dinv,
-                                               T *__restrict__ r, T *__restrict__ z, T *__restrict__ rh,
-                                               T *__restrict__ part, int nb, int *ki, KSeg S) {
+__global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, const int *__restrict__ rp,
+                                               const int *__restrict__ ci, const T *__restrict__ val,
+                                               T *__restrict__ x, const TB *__restrict__ b,
+                                               const T *__restrict__ dinv, T *__restrict__ r, T *__restrict__ z,
+                                               T *__restrict__ rh, T *__restrict__ part, int nb, int *ki, KSeg S) {
     T *ksd = nullptr;
     const KCtx c = kctx<T>(S, nb, ksd, ki);
     const int i = krow<L>();
     const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
-    const T *const vx[1] = {x};
-    T sx[1];
-    row_sums<T, L, 1>(rb, re, ci, val, vx, sx);
+    T sx[1] = {0};
+    if (!xzero) {  // uniform over the launch
+        const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
+        const T *const vx[1] = {x};
+        row_sums<T, L, 1>(rb, re, ci, val, vx, sx);
+    } else if (in && klane<L>() == 0) {
+        x[i] = T(0);
+    }
     T acc[3] = {0, 0, 0};
     if (in && klane<L>() == 0) {
         T bi = (T)b[i];

@@ -814,8 +814,7 @@ class DeviceSystem {
                                          st));
                 k_convert_seg<<<nb, BLOCK, 0, st>>>(n, r.p, b32.p, bseg_.p, segsc_.p);
             }
-            k_fill<float><<<nb, BLOCK, 0, st>>>(n, x32.p, 0.0f);
-            k32.init<float>(spd, b32.p, x32.p, st);
+            k32.init<float>(spd, b32.p, x32.p, st, true);  // x32 = 0, r = b32
             // an inner solve may take at most twice the FP64 solve's count
             const int cap = std::min(maxit - inner_total, std::max(200, 2 * k64.pred + 50));
             // one system: the inner solve reduces its residual by what is
