@@ -463,20 +463,30 @@ int Flow<D>::step(bool ve) {
         // the D Helmholtz systems in the same launches, warm-started from
         // the last u*; one stopping test per component
         bool spd = true;
+        ExtrapSet es{};
+        CopySet cs{};
+        int nmax = 0;
         for (int d = 0; d < D; d++) {
             spd = spd && Hsym_[d];
-            k_extrap<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], X + oUS_[d], hxPrev_.p + hoff_[d],
-                                                                 hxPrev2_.p + hoff_[d], hxB_.p + hoff_[d],
-                                                                 std::min(extrap_, extrap_hist_));
+            es.n[d] = cs.n[d] = Nf_[d];
+            es.cur[d] = X + oUS_[d];
+            es.p1[d] = hxPrev_.p + hoff_[d];
+            es.p2[d] = hxPrev2_.p + hoff_[d];
+            es.out[d] = hxB_.p + hoff_[d];
+            cs.src[d] = hxB_.p + hoff_[d];
+            cs.dst[d] = X + oUS_[d];
+            nmax = std::max(nmax, Nf_[d]);
         }
+        // one launch for the D components: initial guesses, then (after the
+        // solve) the copy back into u*
+        if (nmax > 0) k_extrap_set<<<dim3(nblocks(nmax), D), BLOCK, 0, st_>>>(es, std::min(extrap_, extrap_hist_));
         if (par_.precision == 1)
             hsysB_.fused_mixed(spd, hbB_.p, hxB_.p, par_.rtol, par_.atol, par_.inner_rtol, par_.maxit, st_);
         else
             hsysB_.fused64(spd, hbB_.p, hxB_.p, par_.rtol, par_.atol, par_.maxit, st_);
-        for (int d = 0; d < D; d++) {
-            k_copy<double><<<nblocks(Nf_[d]), BLOCK, 0, st_>>>(Nf_[d], hxB_.p + hoff_[d], X + oUS_[d]);
+        if (nmax > 0) k_copy_set<<<dim3(nblocks(nmax), D), BLOCK, 0, st_>>>(cs);
+        for (int d = 0; d < D; d++)
             if ((int)hsysB_.seg_stats.size() == D) hstats_[d] = hsysB_.seg_stats[d];
-        }
     } else if (semi) {  // one Helmholtz solve per component, warm-started from the last u*
         for (int d = 0; d < D; d++) {
             if (Nf_[d] == 0) continue;

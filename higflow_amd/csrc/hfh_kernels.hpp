@@ -276,6 +276,33 @@ __global__ void __launch_bounds__(BLOCK) k_extrap(int n, const T *cur, T *p1, T 
     p1[i] = v;
 }
 
+// the same for the D velocity components in one launch (blockIdx.y is the
+// component): cur[d], p1[d], p2[d], out[d] of length n[d]
+struct ExtrapSet {
+    int n[3];
+    const double *cur[3];
+    double *p1[3], *p2[3], *out[3];
+};
+__global__ void __launch_bounds__(BLOCK) k_extrap_set(ExtrapSet q, int order) {
+    const int d = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n[d]) return;
+    const double v = q.cur[d][i], a = q.p1[d][i];
+    q.out[d][i] = order == 0 ? v : order == 1 ? 2.0 * v - a : 3.0 * (v - a) + q.p2[d][i];
+    q.p2[d][i] = a;
+    q.p1[d][i] = v;
+}
+
+// dst[d] = src[d] for the D velocity components in one launch
+struct CopySet {
+    int n[3];
+    const double *src[3];
+    double *dst[3];
+};
+__global__ void __launch_bounds__(BLOCK) k_copy_set(CopySet q) {
+    const int d = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < q.n[d]) q.dst[d][i] = q.src[d][i];
+}
+
 // p += dp over all cells (hig-flow-step.c:311-340)
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_axpy_field(int n, T *X, int dst, int src) {
