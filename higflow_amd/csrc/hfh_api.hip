@@ -380,6 +380,29 @@ int hfh_bc_set_values(hfh_solver *s, int kind, int i, const double *v) {
     HFH_SOLVER(s);
     return guard([&] { return s->f->bc_set_values(kind, i, v); });
 }
+int hfh_set_functions(hfh_solver *s, const hfh_functions *f) {
+    HFH_SOLVER(s);
+    hfh::Functions fn;
+    if (f) {
+        fn.get_boundary_pressure = f->get_boundary_pressure;
+        fn.get_boundary_velocity = f->get_boundary_velocity;
+        fn.get_source_term = f->get_source_term;
+        fn.get_facet_source_term = f->get_facet_source_term;
+    }
+    return guard([&] {
+        int r = s->f->set_functions(fn);
+        if (r < 0) g_err = "hfh_set_functions: not set up, or a batch";
+        return r;
+    });
+}
+int hfh_bc_set_valuetype(hfh_solver *s, int kind, int i, int timedependent) {
+    HFH_SOLVER(s);
+    return guard([&] {
+        int r = s->f->bc_set_valuetype(kind, i, timedependent);
+        if (r < 0) g_err = "hfh_bc_set_valuetype: no such boundary, not set up, or a batch";
+        return r;
+    });
+}
 int hfh_field_size(hfh_solver *s, int field, int comp) {
     HFH_SOLVER(s);
     return guard([&] { return s->f->field_size(field, comp); });

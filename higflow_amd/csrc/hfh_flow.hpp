@@ -40,6 +40,15 @@ struct BcSpec {
     int utypes[3];
 };
 
+// The driver callbacks a step re-evaluates on the host (higflow_functions,
+// higflow/src/hig-flow-kernel.h:216-233); a null member is not evaluated.
+struct Functions {
+    double (*get_boundary_pressure)(int id, double *center, double t) = nullptr;
+    double (*get_boundary_velocity)(int id, double *center, int dim, double t) = nullptr;
+    double (*get_source_term)(double *center, double t) = nullptr;
+    double (*get_facet_source_term)(double *center, int dim, double t) = nullptr;
+};
+
 class FlowBase {
   public:
     virtual ~FlowBase() {}
@@ -58,6 +67,10 @@ class FlowBase {
     virtual int bc_count(int kind, int i) = 0;
     virtual int bc_centers(int kind, int i, double *xyz) = 0;
     virtual int bc_set_values(int kind, int i, const double *vals) = 0;
+    // time-dependent boundaries (kind 0 pressure, 1+d velocity component d)
+    // take their values from the callbacks at every step
+    virtual int set_functions(const Functions &f) = 0;
+    virtual int bc_set_valuetype(int kind, int i, int timedependent) = 0;
     virtual int step(bool ve) = 0;
     virtual int stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn,
                         int *ids, double *w, double *rhs) = 0;
@@ -149,6 +162,17 @@ class Flow : public FlowBase {
         bc_dirty_ = true;
         return (int)b->cells.size();
     }
+    int set_functions(const Functions &f) override {
+        if (!ready_ || !mem_.empty()) return -1;
+        fn_ = f;
+        return 0;
+    }
+    int bc_set_valuetype(int kind, int i, int timedependent) override {
+        if (!bc_of(kind, i) || !mem_.empty()) return -1;
+        if (timedep_.empty()) timedep_.assign((size_t)(D + 1) * specs_.size(), 0);
+        timedep_[(size_t)kind * specs_.size() + i] = timedependent != 0;
+        return 0;
+    }
     int step(bool ve) override;
     int stencil(int domain, int dim, const double *center, const double *x, double alpha, int maxn, int *ids,
                 double *w, double *rhs) override;
@@ -224,6 +248,11 @@ class Flow : public FlowBase {
     std::unique_ptr<Domain<D>> sdp_, sdF_, sdED_, sfdu_[D], sfdF_[D];
     std::vector<std::unique_ptr<Boundary<D>>> bcp_, bcu_[D];
     std::vector<double> bcval_;
+    Functions fn_;
+    std::vector<char> timedep_;                  // [kind * nspecs + i]
+    std::vector<double> srcC_, srcV_;            // source-term centres and values (host)
+    void boundary_refresh();
+    void source_refresh();
 
     // state layout
     int Nc_ = 0, Nf_[D] = {0}, ne_ = 0;

@@ -335,20 +335,24 @@ void Flow<D>::refold_constants() {
         DGroup *d;
         Group *g;
     } gs[] = {{&dDu_, &gDu_}, {&dK_, &gK_}, {&dCv_, &gCv_}, {&dMom_, &gMom_}, {&dDiv_, &gDiv_}, {&dGrad_, &gGrad_}};
+    // the host copies stay alive until the stream has consumed them
+    std::vector<std::vector<double>> keep;
+    std::vector<std::vector<float>> keep32;
     for (auto &p : gs) {
         if (p.d->nrow == 0) continue;
-        std::vector<double> c = p.g->constants(bcval_);
+        keep.push_back(p.g->constants(bcval_));
+        const std::vector<double> &c = keep.back();
         if (p.d->f32) {
-            std::vector<float> c32(c.begin(), c.end());
-            p.d->cst32.upload(c32.data(), c32.size(), st_);
-            HFH_CHECK(hipStreamSynchronize(st_));
+            keep32.emplace_back(c.begin(), c.end());
+            p.d->cst32.upload(keep32.back().data(), keep32.back().size(), st_);
         } else {
             p.d->cst.upload(c.data(), c.size(), st_);
         }
     }
     if (par_.tempdiscrtype == SEMI_IMPLICIT_EULER) {
         for (int d = 0; d < D; d++) {
-            std::vector<double> c(Nf_[d]);
+            keep.emplace_back(Nf_[d]);
+            std::vector<double> &c = keep.back();
             for (int f = 0; f < Nf_[d]; f++) {
                 double r = Hrhs0_[d][f];
                 for (auto &b : Hbc_[d][f]) r += b.second * bcval_[b.first];
@@ -359,6 +363,63 @@ void Flow<D>::refold_constants() {
     }
     HFH_CHECK(hipStreamSynchronize(st_));
     bc_dirty_ = false;
+}
+
+// hig-flow-step.c:402-512: boundaries marked time-dependent take the
+// driver's values at t + dt (the pressure's increment over the step for the
+// incremental projection).  The source-term domains carry no boundary
+// trees here, so the source-boundary refresh (:515-606) has nothing to do.
+template <int D>
+void Flow<D>::boundary_refresh() {
+    if (timedep_.empty()) return;
+    const size_t ns = specs_.size();
+    const double t = par_.t, t1 = par_.t + par_.dt;
+    double c[3] = {0, 0, 0};
+    auto fill = [&](Boundary<D> &b, auto &&value) {
+        for (size_t k = 0; k < b.cells.size(); k++) {
+            Vec<D> x = b.tree->center(b.cells[k]);
+            for (int d = 0; d < D; d++) c[d] = x[d];
+            bcval_[b.base + k] = value();
+        }
+        bc_dirty_ = true;
+    };
+    for (int dim = 0; dim < D && fn_.get_boundary_velocity; dim++)
+        for (size_t i = 0; i < ns; i++)
+            if (timedep_[(1 + dim) * ns + i])
+                fill(*bcu_[dim][i], [&] { return fn_.get_boundary_velocity(specs_[i].userid, c, dim, t1); });
+    for (size_t i = 0; i < ns && fn_.get_boundary_pressure; i++)
+        if (timedep_[i])
+            fill(*bcp_[i], [&] {
+                double v = fn_.get_boundary_pressure(specs_[i].userid, c, t1);
+                if (par_.projtype != NON_INCREMENTAL) v -= fn_.get_boundary_pressure(specs_[i].userid, c, t);
+                return v;
+            });
+}
+
+// hig-flow-step.c:343-399: the cell and facet source terms at t, every step,
+// for the callbacks the driver registered; uploaded into F and FU.
+template <int D>
+void Flow<D>::source_refresh() {
+    if (!fn_.get_source_term && !fn_.get_facet_source_term) return;
+    const double t = par_.t;
+    double c[3] = {0, 0, 0};
+    auto run = [&](int field, int comp, auto &&value) {
+        const int n = field_size(field, comp);
+        if (n <= 0) return;
+        srcC_.resize((size_t)n * D);
+        srcV_.resize(n);
+        field_centers(field, comp, srcC_.data());
+        for (int i = 0; i < n; i++) {
+            for (int d = 0; d < D; d++) c[d] = srcC_[(size_t)i * D + d];
+            srcV_[i] = value();
+        }
+        HFH_CHECK(hipMemcpyAsync(X_.p + field_offset(field, comp), srcV_.data(), sizeof(double) * n,
+                                 hipMemcpyHostToDevice, st_));
+        HFH_CHECK(hipStreamSynchronize(st_));
+    };
+    if (fn_.get_source_term) run(4, 0, [&] { return fn_.get_source_term(c, t); });
+    for (int dim = 0; dim < D && fn_.get_facet_source_term; dim++)
+        run(5, dim, [&] { return fn_.get_facet_source_term(c, dim, t); });
 }
 
 template <int D>
@@ -400,6 +461,10 @@ int Flow<D>::step(bool ve) {
     if (par_.tempdiscrtype != EXPLICIT_EULER && par_.tempdiscrtype != SEMI_IMPLICIT_EULER) return -3;
     const bool semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
     if (ve != (par_.flowtype == VISCOELASTIC)) return -4;
+    if (!ve) {
+        boundary_refresh();
+        source_refresh();
+    }
     if (bc_dirty_) refold_constants();
     double *X = X_.p;
     int ne = ne_;
@@ -431,6 +496,11 @@ int Flow<D>::step(bool ve) {
             k_constitutive<D><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
         HFH_CHECK(hipMemcpyAsync(X + oK_, X + oKN_, sizeof(double) * NSYM * ne, hipMemcpyDeviceToDevice, st_));
         tend(ST_VE);
+        // the viscoelastic step refreshes boundary values and sources after
+        // the tensor updates (hig-flow-step-viscoelastic.c:2112-2125)
+        boundary_refresh();
+        source_refresh();
+        if (bc_dirty_) refold_constants();
     }
     tbegin(ST_INTERP);
     run_group(dMom_, 1.0, X, ST_INTERP);

@@ -39,6 +39,36 @@ class Params(C.Structure):
 _LIB = None
 
 
+BP_FN = C.CFUNCTYPE(C.c_double, C.c_int, C.POINTER(C.c_double), C.c_double)
+BV_FN = C.CFUNCTYPE(C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_double)
+SRC_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_double)
+FSRC_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_int, C.c_double)
+
+
+class Functions(C.Structure):
+    """hfh_functions: the higflow_functions callbacks a step re-evaluates."""
+    _fields_ = [("get_boundary_pressure", BP_FN), ("get_boundary_velocity", BV_FN),
+                ("get_source_term", SRC_FN), ("get_facet_source_term", FSRC_FN)]
+
+
+def make_functions(dim, boundary_pressure=None, boundary_velocity=None, source=None, facet_source=None):
+    """Wrap Python callables as hfh_functions: boundary_pressure(id, x, t),
+    boundary_velocity(id, x, d, t), source(x, t), facet_source(x, d, t),
+    with x a numpy copy of the point.  Keep the returned object alive while
+    the solver may call it."""
+    pt = lambda c: np.ctypeslib.as_array(c, shape=(dim,)).copy()
+    f = Functions()
+    if boundary_pressure:
+        f.get_boundary_pressure = BP_FN(lambda i, c, t: float(boundary_pressure(i, pt(c), t)))
+    if boundary_velocity:
+        f.get_boundary_velocity = BV_FN(lambda i, c, d, t: float(boundary_velocity(i, pt(c), d, t)))
+    if source:
+        f.get_source_term = SRC_FN(lambda c, t: float(source(pt(c), t)))
+    if facet_source:
+        f.get_facet_source_term = FSRC_FN(lambda c, d, t: float(facet_source(pt(c), d, t)))
+    return f
+
+
 def load():
     """Load libhigflow_hip.so (raises HipError when it is not built)."""
     global _LIB
@@ -79,6 +109,8 @@ def load():
     L.hfh_bc_count.argtypes = [vp, C.c_int, C.c_int]
     L.hfh_bc_centers.argtypes = [vp, C.c_int, C.c_int, dp]
     L.hfh_bc_set_values.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.hfh_set_functions.argtypes = [vp, C.POINTER(Functions)]
+    L.hfh_bc_set_valuetype.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.hfh_field_size.argtypes = [vp, C.c_int, C.c_int]
     for f in ("hfh_field_centers", "hfh_get_field", "hfh_set_field"):
         getattr(L, f).argtypes = [vp, C.c_int, C.c_int, dp]
@@ -185,6 +217,23 @@ class Flow:
         if n:
             self.L.hfh_bc_centers(self.h, kind, i, _dp(out))
         return out
+
+    def set_bc(self, kind, i, values):
+        """Boundary values of boundary tree i at its cell centres (kind 0
+        pressure, 1+d velocity component d), as sb_set_value does."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != len(self.bc_centers(kind, i)):
+            raise ValueError("boundary value count mismatch")
+        _check(self.L, self.L.hfh_bc_set_values(self.h, kind, i, _dp(v)), "hfh_bc_set_values")
+
+    def set_functions(self, **callables):
+        """Register the driver callbacks (see make_functions) that every step
+        re-evaluates on the host."""
+        self._functions = make_functions(self.dim, **callables)
+        _check(self.L, self.L.hfh_set_functions(self.h, C.byref(self._functions)), "hfh_set_functions")
+
+    def set_valuetype(self, kind, i, timedependent):
+        _check(self.L, self.L.hfh_bc_set_valuetype(self.h, kind, i, int(timedependent)), "hfh_bc_set_valuetype")
 
     def size(self, field, comp=0):
         return _check(self.L, self.L.hfh_field_size(self.h, field, comp), "hfh_field_size")

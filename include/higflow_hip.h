@@ -119,6 +119,31 @@ int hfh_bc_count(hfh_solver *s, int kind, int bcindex);
 int hfh_bc_centers(hfh_solver *s, int kind, int bcindex, double *xyz);
 int hfh_bc_set_values(hfh_solver *s, int kind, int bcindex, const double *vals);
 
+/* The driver callbacks of higflow_functions (higflow/src/hig-flow-kernel.h:
+ * 216-233) that higflow_solver_step re-evaluates every step; Point arguments
+ * are real[DIM], so the driver's own functions fit these pointers.  On each
+ * step, on the host, before the intermediate velocity (after the tensor
+ * updates in the viscoelastic step):
+ *  - boundaries marked time-dependent with hfh_bc_set_valuetype take
+ *    get_boundary_velocity(userid, centre, dim, t + dt) and
+ *    get_boundary_pressure(userid, centre, t + dt) (minus its value at t for
+ *    the incremental projection), hig-flow-step.c:402-512;
+ *  - F = get_source_term(cell centre, t) and FU = get_facet_source_term(
+ *    facet centre, dim, t) on every cell and facet when the pointer is
+ *    non-null, hig-flow-step.c:343-399.
+ * A null member is never called; f = NULL clears them all.  Not for a batch. */
+typedef struct hfh_functions {
+    double (*get_boundary_pressure)(int id, double *center, double t);
+    double (*get_boundary_velocity)(int id, double *center, int dim, double t);
+    double (*get_source_term)(double *center, double t);
+    double (*get_facet_source_term)(double *center, int dim, double t);
+} hfh_functions;
+int hfh_set_functions(hfh_solver *s, const hfh_functions *f);
+/* valuetype of one boundary for one kind (0 pressure, 1+d velocity
+ * component d): 0 fixed (sb_set_value by the driver only), 1 time-dependent
+ * (bc_valuetype timedependent, hig-flow-bc.c:12-26) */
+int hfh_bc_set_valuetype(hfh_solver *s, int kind, int bcindex, int timedependent);
+
 /* fields: 0 u[comp], 1 p, 2 deltap, 3 ustar[comp], 4 F, 5 FU[comp],
  *         6 kernel[i*DIM+j], 7 S[i*DIM+j], 8 Du[i*DIM+j]
  * in the reference's local-id order (cells: depth-first leaves; facets:

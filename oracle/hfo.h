@@ -63,6 +63,19 @@ int hfo_bc_centers(hfo_solver *s, int kind, int bcindex, double *xyz);
 int hfo_bc_set_values(hfo_solver *s, int kind, int bcindex, const double *vals);
 int hfo_bc_userid(hfo_solver *s, int bcindex);
 
+/* higflow_functions (higflow/src/hig-flow-kernel.h:216-233), the callbacks
+ * the step re-evaluates: boundary values of boundaries marked time-dependent
+ * (hfo_bc_set_valuetype; kind 0 pressure, 1+d velocity component d) and,
+ * when set, the cell and facet source terms.  Point arguments are real[DIM]. */
+typedef struct hfo_functions {
+    double (*get_boundary_pressure)(int id, double *center, double t);
+    double (*get_boundary_velocity)(int id, double *center, int dim, double t);
+    double (*get_source_term)(double *center, double t);
+    double (*get_facet_source_term)(double *center, int dim, double t);
+} hfo_functions;
+int hfo_set_functions(hfo_solver *s, const hfo_functions *f);
+int hfo_bc_set_valuetype(hfo_solver *s, int kind, int bcindex, int timedependent);
+
 /* one projection step, then t += dt (ns-example-2d.c:240-246) */
 int hfo_step(hfo_solver *s);
 int hfo_step_viscoelastic(hfo_solver *s);

@@ -33,6 +33,32 @@ def params_from(case_params):
     return p
 
 
+_BP = C.CFUNCTYPE(C.c_double, C.c_int, C.POINTER(C.c_double), C.c_double)
+_BV = C.CFUNCTYPE(C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_double)
+_SRC = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_double)
+_FSRC = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_int, C.c_double)
+
+
+class Functions(C.Structure):
+    """hfo_functions (oracle/hfo.h): the higflow_functions callbacks."""
+    _fields_ = [("get_boundary_pressure", _BP), ("get_boundary_velocity", _BV),
+                ("get_source_term", _SRC), ("get_facet_source_term", _FSRC)]
+
+
+def make_functions(dim, boundary_pressure=None, boundary_velocity=None, source=None, facet_source=None):
+    pt = lambda c: np.ctypeslib.as_array(c, shape=(dim,)).copy()
+    f = Functions()
+    if boundary_pressure:
+        f.get_boundary_pressure = _BP(lambda i, c, t: float(boundary_pressure(i, pt(c), t)))
+    if boundary_velocity:
+        f.get_boundary_velocity = _BV(lambda i, c, d, t: float(boundary_velocity(i, pt(c), d, t)))
+    if source:
+        f.get_source_term = _SRC(lambda c, t: float(source(pt(c), t)))
+    if facet_source:
+        f.get_facet_source_term = _FSRC(lambda c, d, t: float(facet_source(pt(c), d, t)))
+    return f
+
+
 _LIBS = {}
 
 
@@ -65,6 +91,8 @@ def lib(dim, variant=""):
     L.hfo_bc_count.argtypes = [vp, C.c_int, C.c_int]
     L.hfo_bc_centers.argtypes = [vp, C.c_int, C.c_int, dp]
     L.hfo_bc_set_values.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.hfo_set_functions.argtypes = [vp, vp]
+    L.hfo_bc_set_valuetype.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.hfo_step.argtypes = [vp]
     L.hfo_step_viscoelastic.argtypes = [vp]
     L.hfo_advance.argtypes = [vp, C.c_int]
@@ -144,6 +172,20 @@ class Oracle:
         if n > 0:
             self.L.hfo_bc_centers(self.h, kind, i, _dp(out))
         return out
+
+    def set_bc(self, kind, i, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == len(self.bc_centers(kind, i))
+        self.L.hfo_bc_set_values(self.h, kind, i, _dp(v))
+
+    def set_functions(self, **callables):
+        """The same callbacks as higflow_amd.native.Flow.set_functions (the
+        structs have one layout)."""
+        self._functions = make_functions(self.dim, **callables)
+        self.L.hfo_set_functions(self.h, C.addressof(self._functions))
+
+    def set_valuetype(self, kind, i, timedependent):
+        assert self.L.hfo_bc_set_valuetype(self.h, kind, i, int(timedependent)) == 0
 
     def step(self):
         if self.case["params"].get("flowtype", 0) == 3:

@@ -65,6 +65,8 @@ struct hfo_solver {
     lsys ps;
     lsys us[DIM]; /* semi-implicit velocity systems (slvu[dim]) */
     int pin;
+    hfo_functions fn;          /* higflow_functions callbacks, or zeros */
+    char *timedep;             /* [(DIM + 1) * nspecs]: kind-major valuetype */
 };
 
 /* ------------------------------------------------------------ solver */
@@ -1237,6 +1239,7 @@ void hfo_destroy(hfo_solver *s)
     if (!s) return;
     for (int i = 0; i < s->nspecs; i++) free(s->specs[i].text);
     free(s->specs);
+    free(s->timedep);
     if (s->ready) {
         hfo_sd_destroy(s->sdp);
         hfo_sd_destroy(s->sdF);
@@ -1511,11 +1514,82 @@ int hfo_bc_userid(hfo_solver *s, int i)
     return s->specs[i].userid;
 }
 
-/* higflow/src/hig-flow-step.c:1363-1413 (fixed BC values, zero sources:
- * the time-dependent refresh loops have nothing to update) */
+int hfo_set_functions(hfo_solver *s, const hfo_functions *f)
+{
+    if (!s->ready) return -1;
+    if (f) s->fn = *f;
+    else memset(&s->fn, 0, sizeof s->fn);
+    return 0;
+}
+
+int hfo_bc_set_valuetype(hfo_solver *s, int kind, int i, int timedependent)
+{
+    if (!bc_of(s, kind, i)) return -1;
+    if (!s->timedep) s->timedep = calloc((size_t)(DIM + 1) * (size_t)s->nspecs, 1);
+    s->timedep[kind * s->nspecs + i] = (char)(timedependent != 0);
+    return 0;
+}
+
+/* higflow/src/hig-flow-step.c:402-512: time-dependent boundary values.
+ * Velocity at t + dt; pressure at t + dt, or its increment over the step
+ * for the incremental projection.  The cell and facet source domains of
+ * this restatement carry no boundary trees, so :515-606 update nothing. */
+static void boundary_refresh(hfo_solver *s)
+{
+    if (!s->timedep) return;
+    const real t = s->par.t, t1 = s->par.t + s->par.dt;
+    for (int dim = 0; dim < DIM; dim++) {
+        if (!s->fn.get_boundary_velocity) break;
+        for (int i = 0; i < s->nspecs; i++) {
+            if (!s->timedep[(1 + dim) * s->nspecs + i]) continue;
+            hboundary *b = s->bc_u[dim][i];
+            for (int k = 0; k < b->nvals; k++) {
+                Point c;
+                hfo_center(b->cells[k], c);
+                b->vals[k] = s->fn.get_boundary_velocity(s->specs[i].userid, c, dim, t1);
+            }
+        }
+    }
+    if (!s->fn.get_boundary_pressure) return;
+    for (int i = 0; i < s->nspecs; i++) {
+        if (!s->timedep[i]) continue;
+        hboundary *b = s->bc_p[i];
+        for (int k = 0; k < b->nvals; k++) {
+            Point c;
+            hfo_center(b->cells[k], c);
+            real v = s->fn.get_boundary_pressure(s->specs[i].userid, c, t1);
+            if (s->par.projtype != NON_INCREMENTAL) v -= s->fn.get_boundary_pressure(s->specs[i].userid, c, t);
+            b->vals[k] = v;
+        }
+    }
+}
+
+/* higflow/src/hig-flow-step.c:343-399: cell and facet source terms at t,
+ * recomputed every step when the driver registered them */
+static void source_refresh(hfo_solver *s)
+{
+    const real t = s->par.t;
+    if (s->fn.get_source_term)
+        for (int cl = 0; cl < s->sdF->ncells; cl++) {
+            Point c;
+            hfo_center(s->sdF->cells[cl], c);
+            s->F[cl] = s->fn.get_source_term(c, t);
+        }
+    if (s->fn.get_facet_source_term)
+        for (int dim = 0; dim < DIM; dim++)
+            for (int fl = 0; fl < s->sfdF[dim]->nfacets; fl++) {
+                Point c;
+                hfo_facet_center(&s->sfdF[dim]->facets[fl], c);
+                s->FU[dim][fl] = s->fn.get_facet_source_term(c, dim, t);
+            }
+}
+
+/* higflow/src/hig-flow-step.c:1363-1413 */
 int hfo_step(hfo_solver *s)
 {
     if (!s->ready) return -1;
+    boundary_refresh(s);
+    source_refresh(s);
     if (s->par.tempdiscrtype == EXPLICIT_EULER) explicit_euler_intermediate_velocity(s, false);
     else if (s->par.tempdiscrtype == SEMI_IMPLICIT_EULER) semi_implicit_euler_intermediate_velocity(s, false);
     else return -3;
@@ -1534,6 +1608,10 @@ int hfo_step_viscoelastic(hfo_solver *s)
     velocity_derivative_tensor(s);
     explicit_constitutive(s);
     polymeric_tensor(s);
+    /* the boundary and source refresh follows the tensor updates here
+     * (hig-flow-step-viscoelastic.c:2112-2125) */
+    boundary_refresh(s);
+    source_refresh(s);
     if (s->par.tempdiscrtype == EXPLICIT_EULER) explicit_euler_intermediate_velocity(s, true);
     else semi_implicit_euler_intermediate_velocity(s, true);
     pressure(s);

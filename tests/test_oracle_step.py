@@ -64,3 +64,25 @@ def test_projection_is_divergence_free():
     V[:, 1:n] = v
     div = (U[1:] - U[:-1]) / h + (V[:, 1:] - V[:, :-1]) / h
     assert np.abs(div).max() < 1e-8
+
+
+def test_time_dependent_refresh_uses_t_plus_dt():
+    # hig-flow-step.c:402-512: a time-dependent boundary takes its value at
+    # t + dt at the start of the step; sources are evaluated at t
+    # (:343-399).  The same run with the values set by hand must agree
+    # bitwise.
+    g = lambda x, t: np.sin(np.pi * x[0]) * (1.0 + 30.0 * t)
+    case = cases.cavity(12, Re=50.0, rtol=1e-12, atol=1e-14)
+    a, b = hfo.Oracle(case), hfo.Oracle(case)
+    a.set_functions(boundary_velocity=lambda i, x, d, t: g(x, t) if (i == 3 and d == 0) else 0.0,
+                    source=lambda x, t: x[0] * t)
+    a.set_valuetype(1, 3, 1)
+    xl = b.bc_centers(1, 3)
+    for k in range(3):
+        t = b.time()
+        b.set_bc(1, 3, np.array([g(x, t + 1e-3) for x in xl]))
+        b.set(4, 0, b.centers(4)[:, 0] * t)
+        a.step()
+        b.step()
+    for f, c in ((0, 0), (0, 1), (1, 0), (4, 0)):
+        assert np.array_equal(a.get(f, c), b.get(f, c)), (f, c)
