@@ -847,8 +847,11 @@ template <int D>
 int Flow<D>::field_range(int field, int comp, double *mn, double *mx) {
     int n = field_size(field, comp);
     if (n < 0 || host_only_ || !mem_.empty()) return -1;
-    if (!mm_.p) mm_.alloc(2);
-    k_minmax<<<1, 1024, 0, st_>>>(n, X_.p + field_offset(field, comp), mm_.p);
+    const int g = mm_groups(n);
+    if ((long)mm_.n < 2 + 2L * g) mm_.alloc(2 + 2 * (size_t)g);  // min, max, then the partials
+    if (!mmt_.p) mmt_.alloc(1);
+    k_zero_u32<<<1, 64, 0, st_>>>(mmt_.p);
+    k_minmax<<<g, 1024, 0, st_>>>(n, X_.p + field_offset(field, comp), mm_.p + 2, mmt_.p, mm_.p);
     double h[2];
     HFH_CHECK(hipMemcpyAsync(h, mm_.p, sizeof h, hipMemcpyDeviceToHost, st_));
     HFH_CHECK(hipStreamSynchronize(st_));

@@ -324,36 +324,76 @@ __global__ void __launch_bounds__(BLOCK) k_fill(int n, T *x, T v) {
 
 inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
-// out[0] = min, out[1] = max of x[0, n), starting from +-1e16 as the
-// reference's final-velocity loop does (hig-flow-step.c:250-251, 290-291);
-// one workgroup, grid-stride: a diagnostic, not a step kernel
-__global__ void __launch_bounds__(1024) k_minmax(int n, const double *__restrict__ x, double *__restrict__ out) {
-    double mn = 1.0e16, mx = -1.0e16;
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        const double v = x[i];
-        if (v < mn) mn = v;
-        if (v > mx) mx = v;
-    }
+// Min and max of x[0, n) from +-1e16, as the reference's final-velocity
+// loop starts them (hig-flow-step.c:250-251, 290-291).  Workgroup w reduces
+// elements [w MM_CHUNK, (w + 1) MM_CHUNK) into part[2w], part[2w + 1]; the
+// workgroup that finishes last (an agent-scope ticket, no spinning) reduces
+// the partials into out[0], out[1] and rearms the ticket.  Launch
+// mm_groups(n) workgroups of 1024 threads with *ticket == 0.  Min and max
+// are exact, so the result does not depend on the partition.
+constexpr int MM_CHUNK = 8192;
+inline int mm_groups(long n) { return n <= 0 ? 1 : (int)((n + MM_CHUNK - 1) / MM_CHUNK); }
+
+__device__ inline void mm_block_reduce(double &mn, double &mx, double *smn, double *smx) {
     for (int o = 32; o > 0; o >>= 1) {
         const double a = __shfl_down(mn, o, 64), b = __shfl_down(mx, o, 64);
         if (a < mn) mn = a;
         if (b > mx) mx = b;
     }
-    __shared__ double smn[16], smx[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
         smn[w] = mn;
         smx[w] = mx;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
         for (int k = 1; k < 16; k++) {
             if (smn[k] < mn) mn = smn[k];
             if (smx[k] > mx) mx = smx[k];
         }
+}
+
+__global__ void __launch_bounds__(1024) k_minmax(long n, const double *__restrict__ x, double *part,
+                                                   unsigned *ticket, double *__restrict__ out) {
+    __shared__ double smn[16], smx[16];
+    __shared__ int last;
+    const long lo = (long)blockIdx.x * MM_CHUNK;
+    const long hi = lo + MM_CHUNK < n ? lo + MM_CHUNK : n;
+    double mn = 1.0e16, mx = -1.0e16;
+    for (long i = lo + threadIdx.x; i < hi; i += 1024) {
+        const double v = x[i];
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    mm_block_reduce(mn, mx, smn, smx);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&part[2 * blockIdx.x], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[2 * blockIdx.x + 1], mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    mn = 1.0e16;
+    mx = -1.0e16;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += 1024) {
+        const double a = __hip_atomic_load(&part[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double b = __hip_atomic_load(&part[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a < mn) mn = a;
+        if (b > mx) mx = b;
+    }
+    __syncthreads();
+    mm_block_reduce(mn, mx, smn, smx);
+    if (threadIdx.x == 0) {
         out[0] = mn;
         out[1] = mx;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+__global__ void k_zero_u32(unsigned *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
 }  // namespace hfh

@@ -26,26 +26,45 @@ def fields(sim, dim, ve=False):
     return out
 
 
-def compare(case, steps, tol=1e-10, init=None):
+def compare(case, steps, tol=1e-10, init=None, floor=False):
+    """HIP against the oracle after `steps` steps, every field within `tol`
+    of its scale.  floor=True also runs the oracle with its solver
+    tolerances 10x looser and allows each field 10x that self-difference
+    where it exceeds `tol`: where the step amplifies solver round-off (the
+    log-conformation's 1/(lambda_j - lambda_i + 1e-12) near equal
+    eigenvalues, hig-flow-step-viscoelastic.c:1667-1680), two correct
+    solvers differ by more than 1e-10 and the oracle disagrees with itself
+    by as much."""
     o = hfo.Oracle(case)
     h = native.Flow(case)
     dim = case["dim"]
+    runs = [o, h]
+    if floor:
+        loose = dict(case, params=dict(case["params"], rtol=case["params"]["rtol"] * 10,
+                                       atol=case["params"]["atol"] * 10))
+        o2 = hfo.Oracle(loose)
+        runs.append(o2)
     if init:
-        init(o)
-        init(h)
+        for sim in runs:
+            init(sim)
     ve = case["params"]["flowtype"] == 3
     worst = {}
     for k in range(steps):
-        o.step()
-        h.step()
+        for sim in runs:
+            sim.step()
     fo, fh = fields(o, dim, ve), fields(h, dim, ve)
+    f2 = fields(o2, dim, ve) if floor else None
     # relative to the field's scale; velocity components share the velocity
     # scale (a near-zero transverse component is compared against |u|)
     uscale = max(np.abs(fo[("u", d)]).max() for d in range(dim))
+    bad = {}
     for key in fo:
         scale = uscale if key[0] == "u" else np.abs(fo[key]).max()
-        worst[key] = np.abs(fh[key] - fo[key]).max() / scale if scale > 0 else np.abs(fh[key]).max()
-    bad = {k: v for k, v in worst.items() if not v <= tol}
+        rel = lambda a: np.abs(a - fo[key]).max() / scale if scale > 0 else np.abs(a).max()
+        worst[key] = rel(fh[key])
+        allowed = max(tol, 10.0 * rel(f2[key])) if floor else tol
+        if not worst[key] <= allowed:
+            bad[key] = (worst[key], allowed)
     assert not bad, bad
     return worst
 

@@ -124,5 +124,5 @@ def _time_dependent(sim):
 def test_time_dependent_boundaries_and_sources(ve, proj):
     kw = dict(flowtype=cases.VISCOELASTIC, De=0.5, beta=0.5) if ve else {}
     case = cases.newt_channel(nx=40, ny=10, projtype=proj, **kw, **TIGHT)
-    w = compare(case, 4, init=_time_dependent)
+    w = compare(case, 4, init=_time_dependent, floor=ve)
     print(w)
