@@ -2,6 +2,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 
 #include "../../include/higflow_hip.h"
@@ -26,6 +27,38 @@ int guard(F &&f) {
 int fail(const char *m) {
     g_err = m;
     return -1;
+}
+
+// Scratch of hfh_range_noalloc: the k_minmax ticket (first 256 bytes), then
+// one (min, max) pair per workgroup for up to HFH_RANGE_NOALLOC_MAX elements.
+// Each device holds its own copy for the life of the process; its address
+// is looked up once per device with that device current and kept, since
+// the array's name is not a device address in host code.
+constexpr int RANGE_MAX_GROUPS = (int)(HFH_RANGE_NOALLOC_MAX / hfh::MM_CHUNK);
+static_assert((long)RANGE_MAX_GROUPS * hfh::MM_CHUNK == HFH_RANGE_NOALLOC_MAX, "whole workgroups");
+__device__ __align__(256) char scratch[256 + 2 * sizeof(double) * RANGE_MAX_GROUPS];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char *g_scratch[MAX_DEVICES] = {};
+
+char *range_scratch() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        g_err = "hfh_range_noalloc: no current device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_err = std::string("hfh_range_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch[dev] = static_cast<char *>(p);
+    }
+    return g_scratch[dev];
 }
 }  // namespace
 
@@ -555,6 +588,24 @@ int hfh_operator_csr(hfh_solver *s, int which, int *n, int *nnz, int *sym, int *
     });
 }
 long hfh_stage_bytes(hfh_solver *s, int stage) { return (s && s->f) ? s->f->bytes(stage) : -1; }
+
+int hfh_range_noalloc(const double *x, long n, double *minmax, void *stream) {
+    if (n < 0 || n > HFH_RANGE_NOALLOC_MAX)
+        return fail("hfh_range_noalloc: n is negative or above HFH_RANGE_NOALLOC_MAX (33,554,432)");
+    if (!minmax || (n > 0 && !x)) return fail("hfh_range_noalloc: null device pointer");
+    return guard([&] {
+        char *sp = range_scratch();
+        if (!sp) return -1;
+        unsigned *ticket = reinterpret_cast<unsigned *>(sp);
+        double *part = reinterpret_cast<double *>(sp + 256);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        hfh::k_zero_u32<<<1, 64, 0, st>>>(ticket);
+        hfh::k_minmax<<<hfh::mm_groups(n), 1024, 0, st>>>(n, x, part, ticket, minmax);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(hipGetErrorString(e));
+        return 0;
+    });
+}
 int hfh_sync(hfh_solver *s) {
     HFH_SOLVER(s);
     return guard([&] {

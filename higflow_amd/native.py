@@ -115,6 +115,7 @@ def load():
     for f in ("hfh_field_centers", "hfh_get_field", "hfh_set_field"):
         getattr(L, f).argtypes = [vp, C.c_int, C.c_int, dp]
     L.hfh_field_range.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    L.hfh_range_noalloc.argtypes = [vp, C.c_long, vp, vp]
     L.hfh_cell_bounds.argtypes = [vp, dp]
     L.hfh_solver_step.argtypes = [vp]
     L.hfh_solver_step_viscoelastic.argtypes = [vp]

@@ -156,6 +156,19 @@ int hfh_set_field(hfh_solver *s, int field, int comp, const double *in);
  * Vmin/Vmax that higflow_final_velocity prints, hig-flow-step.c:250-306);
  * returns the field size.  With HFH_VERBOSE=1 every step prints that line. */
 int hfh_field_range(hfh_solver *s, int field, int comp, double *vmin, double *vmax);
+/* No counterpart in the reference: hfh_field_range for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops
+ * that log the velocity range every step).  min and max of the device array
+ * x[0, n), from +-1e16 as hfh_field_range starts them, written to the device
+ * array minmax[2]; enqueued on `stream` (a hipStream_t; NULL is the null
+ * stream), with no host synchronisation.  It calls no hipMalloc/hipFree:
+ * its partials live in a fixed scratch array of the library (64 KiB per
+ * device, held until the process ends), so n may not exceed
+ * HFH_RANGE_NOALLOC_MAX.  Calls that overlap on one device (other streams
+ * or threads) share that scratch and race.  Returns 0, or <0 with
+ * hfh_last_error() set. */
+#define HFH_RANGE_NOALLOC_MAX 33554432L
+int hfh_range_noalloc(const double *x, long n, double *minmax, void *stream);
 /* lowpoint[DIM] then highpoint[DIM] of every pressure cell (hig_cell,
  * higtree.h:59-75), in local-id order: the geometry the reference's VTK
  * writer prints (hig-flow-io.c:233-265); returns the cell count */
