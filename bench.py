@@ -25,9 +25,12 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
-# Workloads.  The default is the 2D Oldroyd-B planar 4:1 contraction on a
-# 3-level HiG-Tree AMR mesh (the configuration the metric is quoted on); the
-# cavities are smaller and faster cases of the same step.
+# Workloads.  The default is the largest configuration that runs on one GPU:
+# the batched sweep of 64 Oldroyd-B cavities (1.2 M cells).  The 2D
+# Oldroyd-B planar 4:1 contraction on its 3-level HiG-Tree AMR mesh and the
+# cavities are smaller cases of the same step.  `roof` names the kernel the
+# roofline line reports: the one that dominates the workload's kernel trace,
+# as (hfh_probe id, name, form probed).
 WORKLOADS = {
     # channel of length 20, half-width 4 -> 1, base cells 0.1, 3 factor-2
     # levels within 1.0 of the re-entrant corners (47,800 leaves), parabolic
@@ -53,9 +56,14 @@ WORKLOADS = {
     # PCG32 table (sides 32..256, Re log-uniform in [1, 100], Wi in
     # [0.1, 2]; 1,225,216 cells), beta=0.5, dt=5e-4, semi-implicit Euler,
     # all stepped together (one launch per stage)
-    "sweep64_ve_cavity": dict(kind="sweep", count=64, seed=0, dt=5e-4, steps=100),
+    "sweep64_ve_cavity": dict(kind="sweep", count=64, seed=0, dt=5e-4, steps=100,
+                              roof=(12, "k_kcg_spmv<float, 1> (velocity PCG SpMV, mixed form)", "mixed")),
 }
-DEFAULT = "contraction2d_3lev_oldroydb"
+DEFAULT = "sweep64_ve_cavity"
+# launch A of the fused FP64 BiCGStab on the semi-implicit velocity block
+# system (x, r, p updates and v = (A D^-1) p in one launch; the pressure
+# system's on explicit workloads)
+DEFAULT_ROOF = (9, "k_kbcg_a<double>", "fp64")
 
 
 def make_case(spec, **tol):
@@ -103,7 +111,7 @@ def probe(sim, which, reps):
 def cpu_baseline(case, ncells, budget_s=15.0, max_steps=400):
     """The CPU oracle (FP64, BiCGStab + ILU(0), one core) on the first steps
     of the same workload, until `budget_s` seconds of step time have run
-    (a sweep: its simulations in order, one step each per round).  Returns
+    (a sweep: its simulations in order, one step each).  Returns
     (cell-updates/s, steps, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import hfo
@@ -116,7 +124,7 @@ def cpu_baseline(case, ncells, budget_s=15.0, max_steps=400):
         o.step()
         wall += time.perf_counter() - t0
         sims.append(o)
-        cells_done += o.size(1) if hasattr(o, "size") else len(o.get(1))
+        cells_done += o.size(1)
         done += 1
         if wall >= budget_s:
             break
@@ -127,6 +135,48 @@ def cpu_baseline(case, ncells, budget_s=15.0, max_steps=400):
         cells_done += ncells
         done += 1
     return cells_done / wall, done, wall
+
+
+def _oracle_member_step(row):
+    """Worker of the all-core leg: one oracle step of one sweep simulation
+    (side, Re, Wi, dt, solver tolerances); returns (cells, step seconds)."""
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import hfo
+    from higflow_amd import cases
+    n, re, wi, dt, tol = row
+    o = hfo.Oracle(cases.sweep(table=[(n, re, wi)], dt=dt, **tol)[0])
+    t0 = time.perf_counter()
+    o.step()
+    return o.size(1), time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(rows, workers, budget_s=20.0):
+    """The CPU oracle on every host core at once: the simulations of a
+    sweep are independent, so `workers` processes each step whole
+    simulations (one step each; `rows` as _oracle_member_step takes them)
+    until the members whose steps have started within `budget_s` are done.
+    Returns (cell-updates/s over the wall time of the stepping, simulations
+    stepped, seconds), or None without rows (a single simulation: the
+    oracle steps one mesh on one core)."""
+    if not rows:
+        return None
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")  # no GPU state in the workers
+    cells = 0
+    done = 0
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        # completed steps count as they finish; work still running when the
+        # budget ends is not counted, so the rate errs low
+        for n, _ in pool.imap_unordered(_oracle_member_step, rows):
+            cells += n
+            done += 1
+            if time.perf_counter() - t0 >= budget_s:
+                break
+        wall = time.perf_counter() - t0
+        pool.terminate()
+    return cells / wall, done, wall
 
 
 def main():
@@ -184,19 +234,22 @@ def main():
     # second launch of the same iteration.  Explicit workloads probe the
     # pressure system's launch A instead.
     roof = None
-    pr = probe(sim64, 9, 400)
+    rid, rname, rform = spec.get("roof", DEFAULT_ROOF)
+    pr = probe(simmx if rform == "mixed" else sim64, rid, 400)
     if pr:
         ms, by = pr
         ach = by / (ms * 1e-3) / 1e9
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
         if os.path.exists(tf):
-            traffic = json.load(open(tf)).get("k_kbcg_a_double_bytes_per_launch")
+            traffic = json.load(open(tf)).get("roof_kernel_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": traffic,
-                "kernel": "k_kbcg_a<double>", "bytes_per_launch": by, "avg_us": round(ms * 1e3, 3)}
+                "kernel": rname, "bytes_per_launch": by, "avg_us": round(ms * 1e3, 3)}
     kernels = {}
-    for which, name, sim in ((10, "k_kbcg_b<double>", sim64), (6, "k_kbcg_b<double>(pressure)", sim64),
+    for which, name, sim in ((9, "k_kbcg_a<double>", sim64), (11, "k_kcg_spmv<double>(velocity)", sim64),
+                             (12, "k_kcg_spmv<float>(velocity)", simmx),
+                             (10, "k_kbcg_b<double>", sim64), (6, "k_kbcg_b<double>(pressure)", sim64),
                              (7, "k_kbcg_b<float>(pressure)", simmx), (0, "k_spmv_dot<double>", sim64),
                              (1, "k_momentum", sim64), (2, "k_interp(momentum rows)", sim64),
                              (3, "k_interp(divergence rows)", sim64), (4, "k_interp(projection rows)", sim64),
@@ -208,9 +261,22 @@ def main():
     cpu = None
     if not args.no_cpu:
         v, n, wall = cpu_baseline(case, ncells)
-        cpu = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": "%d steps of %s on the CPU oracle (BiCGStab+ILU(0), rtol=atol=1e-10), %.1f s"
-                         % (n, args.workload, wall)}
+        one = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": "%d %s of %s on the CPU oracle (BiCGStab+ILU(0), rtol=atol=1e-10), %.1f s"
+                         % (n, "simulations stepped once" if batch else "steps", args.workload, wall)}
+        cpu = one
+        workers = min(os.cpu_count() or 1, 16)
+        rows = None
+        if batch:
+            from higflow_amd import cases
+            rows = [(n, re, wi, spec["dt"], ref_tol) for (n, re, wi) in cases.sweep_table(spec["count"], spec["seed"])]
+        allc = cpu_baseline_all_cores(rows, workers) if workers > 1 else None
+        if allc:
+            va, na, wa = allc
+            cpu = {"value": round(va, 1), "unit": "cell-updates/s", "cores": workers, "kind": "port",
+                   "sample": "%d simulations of %s stepped once each on %d worker processes of the CPU oracle "
+                             "(BiCGStab+ILU(0), rtol=atol=1e-10), %.1f s" % (na, args.workload, workers, wa),
+                   "one_core": one}
     line = {
         "metric": "cell-updates/s per full time step (mixed vs FP64 GPU) + Poisson solve ms",
         "value": round(cupsmx, 1),

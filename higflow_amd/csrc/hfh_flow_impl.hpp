@@ -689,6 +689,19 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
             *bytes = by;
             return 0;
         }
+        case 11:
+        case 12: {
+            // PCG launch 1 (SpMV) of the semi-implicit velocity block system
+            // in FP64 (11) / FP32 (12): the symmetric velocity systems of
+            // uniform meshes solve by Jacobi-PCG
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (!hbatched_) return -1;
+            float ms = which == 11 ? hsysB_.k64.probe_cg(reps, &by, st_) : hsysB_.k32.probe_cg(reps, &by, st_);
+            *avg_ms = ms;
+            *bytes = by;
+            return 0;
+        }
         case 8: {
             // plain copy of 8 Mi doubles (64 MiB in, 64 MiB out): the
             // reference point that scales the hardware byte counters

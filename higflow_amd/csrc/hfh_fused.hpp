@@ -182,7 +182,61 @@ class FusedKrylov {
         return ms;
     }
 
+    // Times `reps` launches of PCG launch 1 at iteration 1 (stopping test,
+    // p = z + beta p_old, q = A z + beta A p_old; the solve's SpMV launch)
+    // with the system's own segments, on the current z and p with scalars
+    // that neither stop nor break down (r.r = 1 per workgroup above a zero
+    // tolerance, rho_old = 1); writes p2, q and the partials; call between
+    // solves only.  Returns the average ms and the algorithmic bytes of one
+    // launch: CSR structure and values, z and p_old read once, p_new and q
+    // written once, the partials.
+    float probe_cg(int reps, long *bytes, hipStream_t st) {
+        k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
+        k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(1));
+        std::vector<T> hk((size_t)KS_N * nseg, T(0));
+        for (int g = 0; g < nseg; g++) {
+            hk[(size_t)g * KS_N + KS_RR0] = T(1);
+            hk[(size_t)g * KS_N + KS_RHO] = T(1);
+            hk[(size_t)g * KS_N + KS_RHO + 1] = T(1);
+        }
+        if (hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(ks.p, hk.data(), hk.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("probe set-up failed");
+        float ms = 0;
+        switch (lanes) {
+        case 1: ms = time_cg<1>(reps, st); break;
+        case 2: ms = time_cg<2>(reps, st); break;
+        case 4: ms = time_cg<4>(reps, st); break;
+        default: ms = time_cg<8>(reps, st); break;
+        }
+        long nnz = 0;
+        int last = 0;
+        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
+        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 4 * sizeof(T) +
+                 (long)nb * 4 * sizeof(T);
+        k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
+        k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
+        return ms;
+    }
+
   private:
+    template <int L>
+    float time_cg(int reps, hipStream_t st) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            throw std::runtime_error("hipEventCreate failed");
+        (void)hipEventRecord(a, st);
+        for (int rep = 0; rep < reps; rep++)
+            k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), rp, ci, val.p, z.p, p.p, p2.p, q.p,
+                                                 part.p, nb, ks.p, ki.p, S);
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return reps > 0 ? ms / reps : 0.0f;
+    }
     template <int L>
     float time_a(int reps, hipStream_t st) {
         hipEvent_t a, b;

@@ -210,8 +210,9 @@ int hfh_set_timing(hfh_solver *s, int on);
  * pressure BiCGStab SpMV launch in FP64 / FP32, 8 a plain copy of 64 MiB,
  * 9 / 10 the fused FP64 BiCGStab launch A (update + SpMV) / launch B (SpMV)
  * of the semi-implicit velocity block system (the pressure system's when
- * the step has none).  Average ms per launch and the algorithmic bytes of
- * one launch.  Call between steps. */
+ * the step has none), 11 / 12 the fused PCG SpMV launch of that velocity
+ * system in FP64 / FP32.  Average ms per launch and the algorithmic bytes
+ * of one launch.  Call between steps. */
 int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
 int hfh_sync(hfh_solver *s);
 /* the assembled systems as CSR: which 0 pressure (the matrix slv_set_Ai

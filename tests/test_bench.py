@@ -37,3 +37,19 @@ def test_cpu_baseline_times_the_oracle():
     case = bench.make_case(dict(kind="cavity", n=8, Re=100.0, dt=1e-3, steps=3))
     rate, steps, wall = bench.cpu_baseline(case, 64, budget_s=0.2, max_steps=3)
     assert rate > 0 and 1 <= steps <= 3 and wall > 0
+
+
+def test_all_core_leg_steps_sweep_members_in_parallel():
+    from higflow_amd import cases
+    # the seed-0 parameters on small sides keep the oracle steps short
+    rows = [(8, re, wi, 5e-4, dict(rtol=1e-10, atol=1e-10, maxit=1000))
+            for (_, re, wi) in cases.sweep_table(4, 0)]
+    rate, done, wall = bench.cpu_baseline_all_cores(rows, 2, budget_s=60.0)
+    assert rate > 0 and done == 4 and wall > 0
+    assert bench.cpu_baseline_all_cores(None, 2) is None
+
+
+def test_default_workload_is_the_largest_single_gpu_configuration():
+    assert bench.DEFAULT == "sweep64_ve_cavity"
+    rid, name, form = bench.WORKLOADS[bench.DEFAULT]["roof"]
+    assert form == "mixed" and "k_kcg_spmv" in name
