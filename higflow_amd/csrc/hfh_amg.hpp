@@ -643,8 +643,7 @@ __global__ void __launch_bounds__(KNT) k_abcg_a(int n, int it, T *__restrict__ x
 
 template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol, T atol,
-                                                const int *__restrict__ rp, const int *__restrict__ ci,
-                                                const T *__restrict__ val, const T *__restrict__ ph,
+                                                KRows<T> A, const T *__restrict__ ph,
                                                 const T *__restrict__ rh, T *__restrict__ v, T *__restrict__ part,
                                                 int nb, T *ks, int *ki, KSeg S) {
     const KCtx c = kctx<T>(S, nb, ks, ki);
@@ -653,10 +652,9 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
     reduce_partials<T, 2>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
     const int i = krow<L>();
     const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
     const T *const vv[1] = {ph};
     T sv[1];
-    row_sums<T, L, 1>(rb, re, ci, val, vv, sv);
+    row_sums<T, L, 1>(A, i, in, vv, sv);
     const T rhi = in ? rh[i] : T(0);
     kblock_sync();
     if (pv[2] != T(0)) return;
@@ -690,8 +688,7 @@ __global__ void __launch_bounds__(KNT) k_abcg_c(int n, int it, const T *__restri
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_abcg_d(int n, int it, const int *__restrict__ rp, const int *__restrict__ ci,
-                                                const T *__restrict__ val, const T *__restrict__ sh,
+__global__ void __launch_bounds__(KNT) k_abcg_d(int n, int it, KRows<T> A, const T *__restrict__ sh,
                                                 const T *__restrict__ s, const T *__restrict__ rh,
                                                 T *__restrict__ t, T *__restrict__ part, int nb, const int *ki,
                                                 KSeg S) {
@@ -700,10 +697,9 @@ __global__ void __launch_bounds__(KNT) k_abcg_d(int n, int it, const int *__rest
     if (threadIdx.x == 0) s_done = ki[KI_DONE] + ki[KI_BRK + (it & 1)];
     const int i = krow<L>();
     const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
     const T *const vv[1] = {sh};
     T sv[1];
-    row_sums<T, L, 1>(rb, re, ci, val, vv, sv);
+    row_sums<T, L, 1>(A, i, in, vv, sv);
     const T si = in ? s[i] : T(0), rhi = in ? rh[i] : T(0);
     kblock_sync();
     if (s_done) return;

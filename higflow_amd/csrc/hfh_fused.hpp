@@ -8,6 +8,9 @@
 #include <tuple>
 
 #include "hfh_amg.hpp"
+
+#include <cstdint>
+#include <unordered_map>
 #include "hfh_krylov.hpp"
 
 namespace hfh {
@@ -33,6 +36,14 @@ class FusedKrylov {
     std::vector<int> seg_off;
     KBuf<int> d_wseg, d_wbeg;
     KSeg S;  // device view; S.wseg == nullptr for one segment
+    // template rows (see KRows): one id per row, offsets shared by A and
+    // A D^-1, one weight table each
+    KBuf<int> tcode, tcnt, toff;
+    KBuf<T> twv, twh;
+    int ntempl = 0;
+    long templ_rows = 0;
+    long op_bytes_ = 0;  // the operator as one SpMV launch streams it (one matrix)
+    KRows<T> Av, Ah;  // A and A D^-1 as the kernels read them
 
     FusedKrylov() = default;
     FusedKrylov(const FusedKrylov &) = delete;
@@ -60,6 +71,13 @@ class FusedKrylov {
         up(val, tv, st);
         up(vhat, th, st);
         up(dinv, td, st);
+        Av = KRows<T>();
+        Av.rp = rp;
+        Av.ci = ci;
+        Av.val = val.p;
+        Ah = Av;
+        Ah.val = vhat.p;
+        build_templates(hrp, hci, tv, th, st);
         for (KBuf<T> *b : {&r, &rh, &z, &p, &p2, &q, &v, &v2, &s, &t}) b->alloc(n);
         ks.alloc((size_t)KS_N * nseg);
         ki.alloc((size_t)KI_N * nseg);
@@ -96,6 +114,116 @@ class FusedKrylov {
     }
     bool amg_on() const { return amg.ready; }
 
+    // Rows whose (column offset, weight of A, weight of A D^-1) lists
+    // repeat become template rows: a list is kept as a template when at
+    // least KT_MIN rows share it and it has at most KT_MAX entries, up to
+    // KT_TABLE templates.  When fewer than a quarter of the rows would be
+    // template rows, or HFH_TEMPLATES=0, the matrix stays plain CSR.
+    void build_templates(const std::vector<int> &hrp, const std::vector<int> &hci, const std::vector<T> &tv,
+                         const std::vector<T> &th, hipStream_t st) {
+        constexpr int KT_MIN = 16, KT_TABLE = 8192;
+        ntempl = 0;
+        templ_rows = 0;
+        op_bytes_ = (long)hci.size() * (4 + (long)sizeof(T)) + (long)(n + 1) * 4;
+        if (const char *e = std::getenv("HFH_TEMPLATES"))
+            if (std::strcmp(e, "0") == 0) return;
+        struct Key {
+            std::vector<int> off;
+            std::vector<T> a, b;
+            bool operator==(const Key &o) const {
+                if (off != o.off || a.size() != o.a.size()) return false;
+                for (size_t k = 0; k < a.size(); k++)  // bitwise: -0.0 and 0.0 differ
+                    if (std::memcmp(&a[k], &o.a[k], sizeof(T)) || std::memcmp(&b[k], &o.b[k], sizeof(T)))
+                        return false;
+                return true;
+            }
+        };
+        struct Hash {
+            size_t operator()(const Key &k) const {
+                uint64_t h = 1469598103934665603ull;
+                auto mix = [&](const void *p, size_t nb) {
+                    const unsigned char *c = static_cast<const unsigned char *>(p);
+                    for (size_t q = 0; q < nb; q++) h = (h ^ c[q]) * 1099511628211ull;
+                };
+                mix(k.off.data(), k.off.size() * sizeof(int));
+                mix(k.a.data(), k.a.size() * sizeof(T));
+                mix(k.b.data(), k.b.size() * sizeof(T));
+                return (size_t)h;
+            }
+        };
+        std::unordered_map<Key, int, Hash> ids;
+        std::vector<int> cnt, rowkey(n, -1);
+        std::vector<const Key *> keys;
+        for (int i = 0; i < n; i++) {
+            const int len = hrp[i + 1] - hrp[i];
+            if (len > KT_MAX) continue;
+            Key k;
+            for (int q = hrp[i]; q < hrp[i + 1]; q++) {
+                k.off.push_back(hci[q] - i);
+                k.a.push_back(tv[q]);
+                k.b.push_back(th[q]);
+            }
+            auto it = ids.find(k);
+            if (it == ids.end()) {
+                it = ids.emplace(std::move(k), (int)cnt.size()).first;
+                cnt.push_back(0);
+                keys.push_back(&it->first);
+            }
+            cnt[it->second]++;
+            rowkey[i] = it->second;
+        }
+        // the most shared lists first, up to the table size
+        std::vector<int> order(cnt.size());
+        for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+        std::vector<int> tid(cnt.size(), -1);
+        std::vector<int> hcnt, hoff;
+        std::vector<T> hwv, hwh;
+        long covered = 0;
+        for (int q : order) {
+            if (cnt[q] < KT_MIN || ntempl >= KT_TABLE) break;
+            tid[q] = ntempl++;
+            covered += cnt[q];
+            const Key &k = *keys[q];
+            hcnt.push_back((int)k.off.size());
+            for (int e = 0; e < KT_MAX; e++) {
+                const bool has = e < (int)k.off.size();
+                hoff.push_back(has ? k.off[e] : 0);
+                hwv.push_back(has ? k.a[e] : T(0));
+                hwh.push_back(has ? k.b[e] : T(0));
+            }
+        }
+        if (covered * 4 < n) {
+            ntempl = 0;
+            return;
+        }
+        templ_rows = covered;
+        std::vector<int> code(n);
+        long gen_nnz = 0, gen_rows = 0;
+        for (int i = 0; i < n; i++) {
+            code[i] = rowkey[i] >= 0 ? tid[rowkey[i]] : -1;
+            if (code[i] < 0) {
+                gen_nnz += hrp[i + 1] - hrp[i];
+                gen_rows++;
+            }
+        }
+        // row ids, the general rows' CSR (two row pointers each) and the
+        // template table
+        op_bytes_ = (long)n * 4 + gen_nnz * (4 + (long)sizeof(T)) + gen_rows * 8 +
+                    (long)ntempl * (4 + KT_MAX * (4 + (long)sizeof(T)));
+        up_int(tcode, code, st);
+        up_int(tcnt, hcnt, st);
+        up_int(toff, hoff, st);
+        up(twv, hwv, st);
+        up(twh, hwh, st);
+        if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("template upload failed");
+        Av.code = Ah.code = tcode.p;
+        Av.tcnt = Ah.tcnt = tcnt.p;
+        Av.toff = Ah.toff = toff.p;
+        Av.tw = twv.p;
+        Ah.tw = twh.p;
+    }
+
     // r = b - A x and the flags; x is the initial guess.  `cg` asks for
     // Jacobi-PCG (symmetric positive definite systems); with a V-cycle the
     // solve is AMG-BiCGStab in either case.
@@ -123,8 +251,9 @@ class FusedKrylov {
     // Times `reps` launches of BiCGStab launch B, t = (A D^-1)(r - alpha v)
     // (the solve's SpMV launch shape), on the current vectors with the flags
     // cleared; call between solves only.  Returns the average ms and the
-    // algorithmic bytes of one launch: CSR structure and values, r, v and rh
-    // read once, s and t written once, the partials.
+    // algorithmic bytes of one launch: the operator as stored (template row
+    // ids, or CSR structure and values), r, v and rh read once, s and t
+    // written once, the partials.
     float probe(int reps, long *bytes, hipStream_t st) {
         k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
         float ms = 0;
@@ -134,11 +263,7 @@ class FusedKrylov {
         case 4: ms = time_b<4>(reps, st); break;
         default: ms = time_b<8>(reps, st); break;
         }
-        long nnz = 0;
-        int last = 0;
-        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
-        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 5 * sizeof(T) +
-                 (long)nb * 6 * sizeof(T);
+        *bytes = op_bytes_ + (long)n * 5 * sizeof(T) + (long)nb * 6 * sizeof(T);
         return ms;
     }
 
@@ -149,9 +274,9 @@ class FusedKrylov {
     // beta = alpha = 1/2, no breakdown) and write r, p2, v2 and, in place
     // of the caller's x, the CG-only buffer q; call between solves only
     // (the next solve's init rewrites all of them).  Returns the average ms
-    // and the algorithmic bytes of one launch: CSR structure and values;
-    // s, t, p, v, x, D^-1 and rh read once; x, r, p_new and v_new written
-    // once; the partials.
+    // and the algorithmic bytes of one launch: the operator as stored; s, t,
+    // p, v, x, D^-1 and rh read once; x, r, p_new and v_new written once;
+    // the partials.
     float probe_a(int reps, long *bytes, hipStream_t st) {
         k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
         const T slotv[4] = {T(1), T(1), T(2), T(1)};  // t.s, t.t, rh.s, rh.t
@@ -174,11 +299,7 @@ class FusedKrylov {
         case 4: ms = time_a<4>(reps, st); break;
         default: ms = time_a<8>(reps, st); break;
         }
-        long nnz = 0;
-        int last = 0;
-        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
-        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 11 * sizeof(T) +
-                 (long)nb * 6 * sizeof(T);
+        *bytes = op_bytes_ + (long)n * 11 * sizeof(T) + (long)nb * 6 * sizeof(T);
         return ms;
     }
 
@@ -188,7 +309,7 @@ class FusedKrylov {
     // that neither stop nor break down (r.r = 1 per workgroup above a zero
     // tolerance, rho_old = 1); writes p2, q and the partials; call between
     // solves only.  Returns the average ms and the algorithmic bytes of one
-    // launch: CSR structure and values, z and p_old read once, p_new and q
+    // launch: the operator as stored, z and p_old read once, p_new and q
     // written once, the partials.
     float probe_cg(int reps, long *bytes, hipStream_t st) {
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
@@ -209,11 +330,7 @@ class FusedKrylov {
         case 4: ms = time_cg<4>(reps, st); break;
         default: ms = time_cg<8>(reps, st); break;
         }
-        long nnz = 0;
-        int last = 0;
-        if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nnz = last;
-        *bytes = nnz * (4 + (long)sizeof(T)) + (long)(n + 1) * 4 + (long)n * 4 * sizeof(T) +
-                 (long)nb * 4 * sizeof(T);
+        *bytes = op_bytes_ + (long)n * 4 * sizeof(T) + (long)nb * 4 * sizeof(T);
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
         return ms;
@@ -227,7 +344,7 @@ class FusedKrylov {
             throw std::runtime_error("hipEventCreate failed");
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
-            k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), rp, ci, val.p, z.p, p.p, p2.p, q.p,
+            k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), Av, z.p, p.p, p2.p, q.p,
                                                  part.p, nb, ks.p, ki.p, S);
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
@@ -244,7 +361,7 @@ class FusedKrylov {
             throw std::runtime_error("hipEventCreate failed");
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
-            k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), rp, ci, vhat.p, dinv.p, q.p, r.p, rh.p, s.p,
+            k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), Ah, dinv.p, q.p, r.p, rh.p, s.p,
                                                t.p, p.p, v.p, p2.p, v2.p, part.p, nb, ks.p, ki.p, KSeg());
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
@@ -253,6 +370,12 @@ class FusedKrylov {
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         return reps > 0 ? ms / reps : 0.0f;
+    }
+    static void up_int(KBuf<int> &d, const std::vector<int> &h, hipStream_t st) {
+        d.alloc(h.size());
+        if (!h.empty() &&
+            hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+            throw std::runtime_error("hipMemcpyAsync failed");
     }
     static void up(KBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
         d.alloc(h.size());
@@ -267,7 +390,7 @@ class FusedKrylov {
 
     template <typename TB, int L>
     void launch_init(bool cg, bool xzero, const TB *b, T *x, hipStream_t st) {
-        k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, xzero ? 1 : 0, rp, ci, val.p, x, b, dinv.p, r.p, z.p,
+        k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, xzero ? 1 : 0, Av, x, b, dinv.p, r.p, z.p,
                                               rh.p, part.p, nb,
                                               ki.p, S);
     }
@@ -281,26 +404,26 @@ class FusedKrylov {
                 k_abcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, s.p, t.p, ph.p, sh.p, p.p, v.p, part.p, nb, ks.p,
                                                    ki.p, S);
                 amg.apply(p.p, ph.p, kc, st);
-                k_abcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, ph.p, rh.p, v.p, part.p,
+                k_abcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, Av, ph.p, rh.p, v.p, part.p,
                                                    nb, ks.p, ki.p, S);
                 k_abcg_c<T, L><<<nb, KNT, 0, st>>>(n, it, r.p, v.p, s.p, part.p, nb, ks.p, ki.p, S);
                 amg.apply(s.p, sh.p, kc, st);
-                k_abcg_d<T, L><<<nb, KNT, 0, st>>>(n, it, rp, ci, val.p, sh.p, s.p, rh.p, t.p, part.p, nb, ki.p, S);
+                k_abcg_d<T, L><<<nb, KNT, 0, st>>>(n, it, Av, sh.p, s.p, rh.p, t.p, part.p, nb, ki.p, S);
             }
             return;
         }
         for (int it = it0; it < it0 + count; it++) {
             T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
             if (cg) {
-                k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, val.p, z.p, pold, pnew, q.p,
+                k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, Av, z.p, pold, pnew, q.p,
                                                      part.p, nb, ks.p, ki.p, S);
                 k_kcg_update<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb, ks.p, ki.p,
                                                        S);
             } else {
                 T *vold = (it & 1) ? v.p : v2.p, *vnew = (it & 1) ? v2.p : v.p;
-                k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, rp, ci, vhat.p, dinv.p, x, r.p, rh.p, s.p,
+                k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, Ah, dinv.p, x, r.p, rh.p, s.p,
                                                    t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p, S);
-                k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rp, ci, vhat.p, r.p, rh.p, vnew, s.p, t.p, part.p,
+                k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, Ah, r.p, rh.p, vnew, s.p, t.p, part.p,
                                                    nb, ks.p, ki.p, S);
             }
         }
@@ -312,7 +435,7 @@ class FusedKrylov {
             throw std::runtime_error("hipEventCreate failed");
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
-            k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, 0, 1 << 30, rp, ci, vhat.p, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
+            k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, 0, 1 << 30, Ah, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
                                                ks.p, ki.p, KSeg());
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);

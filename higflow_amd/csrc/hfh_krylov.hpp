@@ -203,17 +203,52 @@ __device__ inline int klane() {
     return (int)threadIdx.x % L;
 }
 
-// sums over the row's entries of val[k] * vec[m][ci[k]], split over L lanes
-// and combined so that every lane holds the row totals (fixed order).  Each
-// lane takes its entries four at a time: the four column indices are loaded
-// together, then all their vector values, so a long (WLS) row costs a
-// quarter of the dependent load round trips.
+// A matrix as the Krylov kernels read it: CSR (rp, ci, val) for every row,
+// and optionally template rows.  Rows of a uniform mesh repeat a few
+// stencils; such a row stores only its template id code[i] >= 0, and
+// template t holds tcnt[t] <= KT_MAX entries (column offset from the row,
+// weight) at t * KT_MAX in toff / tw, in the row's CSR order.  code[i] < 0,
+// or code == nullptr, reads the CSR row.  Template rows then cost 4 bytes
+// of operator instead of 4 + 12 nnz (FP64), and sum in the same order.
+constexpr int KT_MAX = 16;
+template <typename T>
+struct KRows {
+    const int *rp = nullptr, *ci = nullptr;
+    const T *val = nullptr;
+    const int *code = nullptr, *tcnt = nullptr, *toff = nullptr;
+    const T *tw = nullptr;
+};
+
+// sums over row i's entries of val[k] * vec[m][col[k]], split over L lanes
+// and combined so that every lane holds the row totals (fixed order: lane l
+// takes entries l, l + L, ...; then a butterfly).  Each lane takes its
+// entries four at a time: the four column indices are loaded together,
+// then all their vector values, so a long (WLS) row costs a quarter of the
+// dependent load round trips.  A template row sums the same entries in the
+// same order as its CSR form.
 template <typename T, int L, int M>
-__device__ inline void row_sums(int rb, int re, const int *__restrict__ ci, const T *__restrict__ val,
-                                const T *const (&vec)[M], T (&out)[M]) {
+__device__ inline void row_sums(const KRows<T> &A, int i, bool in, const T *const (&vec)[M], T (&out)[M]) {
     constexpr int U = 4;
 #pragma unroll
     for (int m = 0; m < M; m++) out[m] = 0;
+    const int t = (in && A.code) ? A.code[i] : -1;
+    int rb = 0, re = 0;
+    const int *cp;
+    const T *vp;
+    int base;
+    if (t >= 0) {
+        rb = t * KT_MAX;
+        re = rb + A.tcnt[t];
+        cp = A.toff;
+        vp = A.tw;
+        base = i;
+    } else {
+        rb = in ? A.rp[i] : 0;
+        re = in ? A.rp[i + 1] : 0;
+        cp = A.ci;
+        vp = A.val;
+        base = 0;
+    }
 #pragma unroll 1
     for (int k0 = rb + klane<L>(); k0 < re; k0 += U * L) {
         int j[U];
@@ -223,8 +258,8 @@ __device__ inline void row_sums(int rb, int re, const int *__restrict__ ci, cons
         for (int q = 0; q < U; q++) {
             const int k = k0 + q * L;
             ok[q] = k < re;
-            j[q] = ok[q] ? ci[k] : 0;
-            a[q] = ok[q] ? val[k] : T(0);
+            j[q] = ok[q] ? base + cp[k] : 0;
+            a[q] = ok[q] ? vp[k] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < U; q++)
@@ -245,8 +280,7 @@ __device__ inline void row_sums(int rb, int re, const int *__restrict__ ci, cons
 // Partials r.r and b.b; clears the flags.  xzero: x is set to zero here and
 // r = b (no product; the corrections of a refinement start from zero).
 template <typename T, typename TB, int L>
-__global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, const int *__restrict__ rp,
-                                               const int *__restrict__ ci, const T *__restrict__ val,
+__global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, KRows<T> A,
                                                T *__restrict__ x, const TB *__restrict__ b,
                                                const T *__restrict__ dinv, T *__restrict__ r, T *__restrict__ z,
                                                T *__restrict__ rh, T *__restrict__ part, int nb, int *ki, KSeg S) {
@@ -256,9 +290,8 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, const i
     const bool in = i < n;
     T sx[1] = {0};
     if (!xzero) {  // uniform over the launch
-        const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
         const T *const vx[1] = {x};
-        row_sums<T, L, 1>(rb, re, ci, val, vx, sx);
+        row_sums<T, L, 1>(A, i, in, vx, sx);
     } else if (in && klane<L>() == 0) {
         x[i] = T(0);
     }
@@ -294,8 +327,7 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, const i
 // to pnew), q = A z + beta A p_old; partial p.q
 template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rtol, T atol,
-                                                  const int *__restrict__ rp, const int *__restrict__ ci,
-                                                  const T *__restrict__ val, const T *__restrict__ z,
+                                                  KRows<T> A, const T *__restrict__ z,
                                                   const T *__restrict__ pold, T *__restrict__ pnew,
                                                   T *__restrict__ q, T *__restrict__ part, int nb, T *ks, int *ki,
                                                   KSeg S) {
@@ -305,10 +337,9 @@ __global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rt
     reduce_partials<T, 3>(part, nb, c, sl, ki, pv);
     const int i = krow<L>();
     const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
     const T *const vz[2] = {z, it == 0 ? z : pold};
     T sz[2];
-    row_sums<T, L, 2>(rb, re, ci, val, vz, sz);
+    row_sums<T, L, 2>(A, i, in, vz, sz);
     const T zi = in ? z[i] : T(0), poi = (in && it > 0) ? pold[i] : T(0);
     kblock_sync();
     if (pv[3] != T(0)) return;
@@ -379,8 +410,7 @@ __global__ void __launch_bounds__(KNT) k_kcg_update(int n, int it, T *__restrict
 // p = r + beta (p_old - omega v_old), v = (A D^-1) p.  Partials rh.v, r.r.
 template <typename T, int L>
 __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol, T atol,
-                                                const int *__restrict__ rp, const int *__restrict__ ci,
-                                                const T *__restrict__ vhat, const T *__restrict__ dinv,
+                                                KRows<T> A, const T *__restrict__ dinv,
                                                 T *__restrict__ x, T *__restrict__ r, const T *__restrict__ rh,
                                                 const T *__restrict__ s, const T *__restrict__ t,
                                                 const T *__restrict__ pold, const T *__restrict__ vold,
@@ -390,7 +420,6 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
     const int i = krow<L>();
     const bool in = i < n;
     const bool own = in && klane<L>() == 0;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
     T acc[2] = {0, 0};
     if (it == 0) {
         const int sl[3] = {KP_RZ, KP_RR, KP_BB};
@@ -398,7 +427,7 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
         reduce_partials<T, 3>(part, nb, c, sl, ki, pv);
         const T *const vr[1] = {r};
         T sr[1];
-        row_sums<T, L, 1>(rb, re, ci, vhat, vr, sr);
+        row_sums<T, L, 1>(A, i, in, vr, sr);
         const T ri = in ? r[i] : T(0), rhi = in ? rh[i] : T(0);
         kblock_sync();
         if (pv[3] != T(0)) return;
@@ -416,7 +445,7 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
         reduce_partials<T, 4>(part, nb, c, sl, ki, pv, KI_BRK + ((it + 1) & 1));
         const T *const vv[4] = {s, t, pold, vold};
         T sv[4];
-        row_sums<T, L, 4>(rb, re, ci, vhat, vv, sv);
+        row_sums<T, L, 4>(A, i, in, vv, sv);
         T si = 0, ti = 0, poi = 0, voi = 0, xi = 0, di = 0, rhi = 0;
         if (own) {
             si = s[i];
@@ -468,8 +497,7 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
 // launch B: stopping test on r.r; alpha = rho / rh.v; s = r - alpha v;
 // t = (A D^-1) s; partials t.s, t.t, rh.s, rh.t
 template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, const int *__restrict__ rp,
-                                                const int *__restrict__ ci, const T *__restrict__ vhat,
+__global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, KRows<T> A,
                                                 const T *__restrict__ r, const T *__restrict__ rh,
                                                 const T *__restrict__ v, T *__restrict__ s, T *__restrict__ t,
                                                 T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
@@ -479,10 +507,9 @@ __global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, const 
     reduce_partials<T, 2>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
     const int i = krow<L>();
     const bool in = i < n;
-    const int rb = in ? rp[i] : 0, re = in ? rp[i + 1] : 0;
     const T *const vv[2] = {r, v};
     T sv[2];
-    row_sums<T, L, 2>(rb, re, ci, vhat, vv, sv);
+    row_sums<T, L, 2>(A, i, in, vv, sv);
     const T ri = in ? r[i] : T(0), vi = in ? v[i] : T(0), rhi = in ? rh[i] : T(0);
     kblock_sync();
     if (pv[2] != T(0)) return;
