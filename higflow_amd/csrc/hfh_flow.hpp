@@ -282,6 +282,12 @@ class Flow : public FlowBase {
         DevBuf<int> rp, col, out, y;
         DevBuf<double> val, cst;
         DevBuf<float> val32, cst32;  // FP32 rows (mixed form)
+        // template rows (GTempl): id and column base per row, the table
+        DevBuf<int> tcode, tbase, tcnt, toff;
+        DevBuf<double> tw;
+        DevBuf<float> tw32;
+        int ntempl = 0;
+        long op_bytes = 0;  // the operator as a launch streams it
         int nrow = 0, ncomp = 1, in_stride = 0, out_stride = 0;
         int lanes = 1;  // lanes per row (from the mean row length)
         bool has_y = false, f32 = false;
@@ -388,6 +394,11 @@ class Flow : public FlowBase {
     void finalize_layout();
     void validate() const;
     void upload_group(DGroup &dg, const Group &g, int pbase, int out_stride, bool has_y, bool f32 = false);
+    // one launch of a group: the operator as uploaded, then per (row,
+    // component) the constant, the destination index and the write
+    static long gdbytes(const DGroup &dg) {
+        return dg.nrow ? dg.op_bytes + (long)dg.nrow * dg.ncomp * (8 + 4 + 8) : 0;
+    }
     void refold_constants();
     void run_group(DGroup &dg, double alpha, double *dst, int stage);
     template <typename W>

@@ -172,6 +172,15 @@ void Flow<D>::upload_device() {
         dVeFlags_.upload(veFlags_.data(), veFlags_.size(), st_);
         dVeDelta_.upload(veDelta_.data(), veDelta_.size(), st_);
     }
+    {
+        // before the groups: the FP32 form uploads FP32 weights
+        const char *mk = std::getenv("HFH_MIXED_KERNELS");
+        k32_ = par_.precision == 1 && mk && std::strcmp(mk, "1") == 0;
+        // HFH_VERBOSE=1: print each step's velocity range as the reference's
+        // final-velocity step does (hig-flow-step.c:302-306)
+        const char *ev = std::getenv("HFH_VERBOSE");
+        verbose_ = ev && std::strcmp(ev, "1") == 0;
+    }
     upload_group(dDu_, gDu_, 0, 0, false, k32_);
     upload_group(dK_, gK_, 0, gK_.nrows(), false, k32_);
     upload_group(dCv_, gCv_, 0, 0, false, k32_);
@@ -190,17 +199,13 @@ void Flow<D>::upload_device() {
         }
         upload_group(dGrad_, gGrad_, 0, 0, true);
         dGrad_.y.upload(y.data(), y.size(), st_);
+        HFH_CHECK(hipStreamSynchronize(st_));  // y is a local
     }
-    {
-        const char *mk = std::getenv("HFH_MIXED_KERNELS");
-        k32_ = par_.precision == 1 && mk && std::strcmp(mk, "1") == 0;
-    }
-    {
-        // HFH_VERBOSE=1: print each step's velocity range as the reference's
-        // final-velocity step does (hig-flow-step.c:302-306)
-        const char *ev = std::getenv("HFH_VERBOSE");
-        verbose_ = ev && std::strcmp(ev, "1") == 0;
-    }
+    // algorithmic bytes of the interpolation stages with the operators as
+    // uploaded (template rows or CSR)
+    bytes_[ST_INTERP] = gdbytes(dMom_) + gdbytes(dDu_) + gdbytes(dK_) + gdbytes(dCv_);
+    bytes_[ST_DIV] = gdbytes(dDiv_);
+    bytes_[ST_PROJECT] = gdbytes(dGrad_) + (long)dGrad_.nrow * 12 + (long)Nc_ * 24;
     {
         const char *ex = std::getenv("HFH_EXTRAP");
         extrap_ = ex ? std::max(0, std::min(2, std::atoi(ex))) : 2;
@@ -295,6 +300,9 @@ void Flow<D>::validate() const {
         if (Aci_[k] < 0 || Aci_[k] >= Nc_) fail("pressure matrix column out of range");
 }
 
+static long group_templates(const Group &g, std::vector<int> &code, std::vector<int> &base, std::vector<int> &tcnt,
+                            std::vector<int> &toff, std::vector<double> &tw, long &gen_nnz);
+
 template <int D>
 void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool has_y, bool f32) {
     dg.nrow = g.nrows();
@@ -320,13 +328,152 @@ void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool
     }
     dg.out.upload(g.out.data(), g.out.size(), st_);
     dg.cst.alloc(dg.nrow);
+    // template rows: the operator bytes of repeated stencils shrink to a
+    // template id and a column base per row
+    dg.ntempl = 0;
+    dg.op_bytes = (long)g.col.size() * (4 + (f32 ? 4 : 8)) + (long)(dg.nrow + 1) * 4;
+    double avg = (double)g.col.size() / dg.nrow;
+    {
+        std::vector<int> code, base, tcnt, toff;
+        std::vector<double> tw;
+        long gen_nnz = 0;
+        const long covered = group_templates(g, code, base, tcnt, toff, tw, gen_nnz);
+        if (verbose_)
+            std::fprintf(stderr, "hfh: group of %d rows (%zu entries): %ld template rows, %zu templates\n", dg.nrow,
+                         g.col.size(), covered, tcnt.size());
+        if (covered > 0) {
+            dg.ntempl = (int)tcnt.size();
+            dg.tcode.upload(code.data(), code.size(), st_);
+            dg.tbase.upload(base.data(), base.size(), st_);
+            dg.tcnt.upload(tcnt.data(), tcnt.size(), st_);
+            dg.toff.upload(toff.data(), toff.size(), st_);
+            std::vector<float> tw32(tw.begin(), tw.end());
+            if (f32)
+                dg.tw32.upload(tw32.data(), tw32.size(), st_);
+            else
+                dg.tw.upload(tw.data(), tw.size(), st_);
+            HFH_CHECK(hipStreamSynchronize(st_));
+            const long gen_rows = dg.nrow - covered;
+            dg.op_bytes = (long)dg.nrow * 8 + gen_nnz * (4 + (f32 ? 4 : 8)) + gen_rows * 8 +
+                          (long)dg.ntempl * (4 + GT_MAX * (4 + (f32 ? 4 : 8)));
+            // lanes by the entries a row actually reads
+            long entries = gen_nnz;
+            for (int r = 0; r < dg.nrow; r++)
+                if (code[r] >= 0) entries += tcnt[code[r]];
+            avg = (double)entries / dg.nrow;
+        }
+    }
+    HFH_CHECK(hipStreamSynchronize(st_));  // the host copies above are locals
     // lanes per row from the mean row length (HFH_INTERP_LANES overrides)
-    const double avg = (double)g.col.size() / dg.nrow;
     dg.lanes = avg < 4 ? 1 : avg < 8 ? 2 : avg < 16 ? 4 : avg < 32 ? 8 : 16;
     if (const char *e = std::getenv("HFH_INTERP_LANES")) {
         int f = std::atoi(e);
         if (f == 1 || f == 2 || f == 4 || f == 8 || f == 16) dg.lanes = f;
     }
+}
+
+// Template rows of an interpolation group.  Rows of at most GT_MAX entries
+// are keyed by their column offsets from the first column and by their
+// weights rounded to 20 mantissa bits; a row joins the template of its key
+// when every weight is within 1e-11 of the row's largest weight, and the
+// template takes the weights of the first row that made it.  The WLS
+// weights of congruent point sets are equal in exact arithmetic; computed
+// from absolute coordinates, as the reference computes them, they differ
+// from position to position by rounding of 1e-12 to 1e-11 relative, so a
+// template row's weights match its own to that level (HFH_TEMPLATES=0 keeps
+// every row's own weights).  Templates shared by at least 8 rows are kept, up to
+// 16,384; with fewer than a quarter of the rows covered, or
+// HFH_TEMPLATES=0, the group stays CSR.  Returns the rows covered.
+static long group_templates(const Group &g, std::vector<int> &code, std::vector<int> &base, std::vector<int> &tcnt,
+                            std::vector<int> &toff, std::vector<double> &tw, long &gen_nnz) {
+    constexpr int GT_MIN = 8, GT_TABLE = 16384, QBITS = 20;
+    constexpr double TOL = 1e-11;
+    if (const char *e = std::getenv("HFH_TEMPLATES"))
+        if (std::strcmp(e, "0") == 0) return 0;
+    const int nr = g.nrows();
+    struct Key {
+        std::vector<int> off;
+        std::vector<int64_t> q;
+        bool operator==(const Key &o) const { return off == o.off && q == o.q; }
+    };
+    struct Hash {
+        size_t operator()(const Key &k) const {
+            uint64_t h = 1469598103934665603ull;
+            for (int v : k.off) h = (h ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
+            for (int64_t v : k.q) h = (h ^ (uint64_t)v) * 1099511628211ull;
+            return (size_t)h;
+        }
+    };
+    auto quant = [](double w) -> int64_t {
+        if (w == 0.0) return 0;
+        int ex;
+        const double m = std::frexp(w, &ex);
+        return (int64_t)std::llround(std::ldexp(m, QBITS)) * 4096 + ex;
+    };
+    std::unordered_map<Key, int, Hash> ids;
+    std::vector<int> first, cnt, rowkey(nr, -1);
+    for (int r = 0; r < nr; r++) {
+        const int b = g.rowptr[r], e = g.rowptr[r + 1];
+        if (e - b < 1 || e - b > GT_MAX) continue;
+        Key k;
+        double mx = 0;
+        for (int q = b; q < e; q++) mx = std::max(mx, std::fabs(g.val[q]));
+        for (int q = b; q < e; q++) {
+            k.off.push_back(g.col[q] - g.col[b]);
+            k.q.push_back(quant(g.val[q]));
+        }
+        auto it = ids.find(k);
+        int id;
+        if (it == ids.end()) {
+            id = (int)first.size();
+            ids.emplace(std::move(k), id);
+            first.push_back(r);
+            cnt.push_back(0);
+        } else {
+            id = it->second;
+            const int f = first[id], fb = g.rowptr[f];
+            bool near = true;
+            for (int q = b; q < e && near; q++) near = std::fabs(g.val[q] - g.val[fb + (q - b)]) <= TOL * mx;
+            if (!near) continue;
+        }
+        cnt[id]++;
+        rowkey[r] = id;
+    }
+    std::vector<int> order(cnt.size());
+    for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+    std::vector<int> tid(cnt.size(), -1);
+    long covered = 0;
+    for (int q : order) {
+        if (cnt[q] < GT_MIN || (int)tcnt.size() >= GT_TABLE) break;
+        tid[q] = (int)tcnt.size();
+        covered += cnt[q];
+        const int f = first[q], fb = g.rowptr[f], fe = g.rowptr[f + 1];
+        tcnt.push_back(fe - fb);
+        for (int e = 0; e < GT_MAX; e++) {
+            const bool has = e < fe - fb;
+            toff.push_back(has ? g.col[fb + e] - g.col[fb] : 0);
+            tw.push_back(has ? g.val[fb + e] : 0.0);
+        }
+    }
+    if (covered * 4 < nr) {
+        tcnt.clear();
+        toff.clear();
+        tw.clear();
+        return 0;
+    }
+    code.assign(nr, -1);
+    base.assign(nr, 0);
+    gen_nnz = 0;
+    for (int r = 0; r < nr; r++) {
+        const int t = rowkey[r] >= 0 ? tid[rowkey[r]] : -1;
+        code[r] = t;
+        if (t >= 0)
+            base[r] = g.col[g.rowptr[r]];
+        else
+            gen_nnz += g.rowptr[r + 1] - g.rowptr[r];
+    }
+    return covered;
 }
 
 template <int D>
@@ -437,14 +584,25 @@ template <typename W>
 void Flow<D>::launch_rows(DGroup &dg, const W *val, const W *cst, W alpha, double *dst) {
     long n = (long)dg.nrow * dg.ncomp;
     const int *y = dg.has_y ? dg.y.p : nullptr;
+    GTempl<W> tp;
+    if (dg.ntempl > 0) {
+        tp.code = dg.tcode.p;
+        tp.base = dg.tbase.p;
+        tp.tcnt = dg.tcnt.p;
+        tp.toff = dg.toff.p;
+        if constexpr (sizeof(W) == 4)
+            tp.tw = dg.tw32.p;
+        else
+            tp.tw = dg.tw.p;
+    }
 #define HFH_INTERP_L(L)                                                                                        \
     k_interp_l<double, L, W><<<nblocks(n * L), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, val, cst, \
                                                                 dg.out.p, y, alpha, dg.in_stride, dg.out_stride, \
-                                                                X_.p, dst)
+                                                                X_.p, dst, tp)
     switch (dg.lanes) {
     case 1:
         k_interp<double, W><<<nblocks(n), BLOCK, 0, st_>>>(dg.nrow, dg.ncomp, dg.rp.p, dg.col.p, val, cst, dg.out.p,
-                                                           y, alpha, dg.in_stride, dg.out_stride, X_.p, dst);
+                                                           y, alpha, dg.in_stride, dg.out_stride, X_.p, dst, tp);
         break;
     case 2: HFH_INTERP_L(2); break;
     case 4: HFH_INTERP_L(4); break;
@@ -635,7 +793,6 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
     double *X = X_.p;
     long by = 0;
     int n = Nc_;
-    auto gb = [&](const Group &g) { return (long)g.col.size() * 12 + (long)g.nrows() * (8 + 8 + 4 + 4) * g.ncomp; };
     MomParams mp;
     mp.nf = Nf_[0];
     mp.ns = momNs_;
@@ -734,16 +891,16 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
             break;
         case 2:
             run_group(dMom_, 1.0, X, ST_INTERP);
-            by = gb(gMom_);
+            by = gdbytes(dMom_);
             break;
         case 3:
             run_group(dDiv_, 1.0, bvec_.p, ST_DIV);
-            by = gb(gDiv_);
+            by = gdbytes(dDiv_);
             break;
         case 4:
             // recomputes u = ustar - dt*grad(dp) from unchanged inputs: idempotent
             run_group(dGrad_, -par_.dt, X, ST_PROJECT);
-            by = gb(gGrad_) + (long)gGrad_.nrows() * 12;
+            by = gdbytes(dGrad_) + (long)dGrad_.nrow * 12;
             break;
         default: {
             if (par_.flowtype != VISCOELASTIC) return -1;

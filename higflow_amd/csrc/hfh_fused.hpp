@@ -10,6 +10,7 @@
 #include "hfh_amg.hpp"
 
 #include <cstdint>
+#include <cstdio>
 #include <unordered_map>
 #include "hfh_krylov.hpp"
 
@@ -193,6 +194,10 @@ class FusedKrylov {
                 hwh.push_back(has ? k.b[e] : T(0));
             }
         }
+        if (const char *ev = std::getenv("HFH_VERBOSE"))
+            if (std::strcmp(ev, "1") == 0)
+                std::fprintf(stderr, "hfh: %zu-byte system of %d rows: %ld template rows, %d templates\n",
+                             sizeof(T), n, covered, ntempl);
         if (covered * 4 < n) {
             ntempl = 0;
             return;

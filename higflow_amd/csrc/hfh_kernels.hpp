@@ -14,6 +14,38 @@ constexpr int BLOCK = 256;
 #define HFH_FLT_EQ(x, y) (fabs((x) - (y)) < 1.0e-12)
 
 // ---------------------------------------------------------------- interp
+// Template rows of an interpolation group: rows whose entries repeat a
+// pattern store its id code[r] >= 0 and the column base[r] of their first
+// entry; template t holds tcnt[t] <= GT_MAX entries (column offset from the
+// base, weight) at t * GT_MAX in toff / tw, in the row's order.  code[r] < 0,
+// or code == nullptr, reads the CSR row.
+constexpr int GT_MAX = 64;
+template <typename W>
+struct GTempl {
+    const int *code = nullptr, *base = nullptr, *tcnt = nullptr, *toff = nullptr;
+    const W *tw = nullptr;
+};
+
+// entries [*b, *e) of row r, read through *cp / *vp with column base *cb
+template <typename W>
+__device__ inline void grow(const GTempl<W> &tp, int r, const int *rowptr, const int *col, const W *val, int &b,
+                            int &e, const int *&cp, const W *&vp, int &cb) {
+    const int t = tp.code ? tp.code[r] : -1;
+    if (t >= 0) {
+        b = t * GT_MAX;
+        e = b + tp.tcnt[t];
+        cp = tp.toff;
+        vp = tp.tw;
+        cb = tp.base[r];
+    } else {
+        b = rowptr[r];
+        e = rowptr[r + 1];
+        cp = col;
+        vp = val;
+        cb = 0;
+    }
+}
+
 // Affine rows over the state vector X:
 //   v = cst[r] + sum_k val[k] * X[col[k] + c*in_stride]
 //   dst[out[r] + c*out_stride] = alpha * v + (yoff ? X[yoff[r]] : 0)
@@ -26,14 +58,17 @@ __global__ void __launch_bounds__(BLOCK) k_interp(int nrow, int ncomp, const int
                                                   const int *__restrict__ col, const W *__restrict__ val,
                                                   const W *__restrict__ cst, const int *__restrict__ out,
                                                   const int *__restrict__ yoff, W alpha, int in_stride,
-                                                  int out_stride, const T *X, T *dst) {
+                                                  int out_stride, const T *X, T *dst, GTempl<W> tp) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nrow * ncomp) return;
     int r = t % nrow, c = t / nrow;
-    int b = rowptr[r], e = rowptr[r + 1];
+    int b, e, cb;
+    const int *cp;
+    const W *vp;
+    grow<W>(tp, r, rowptr, col, val, b, e, cp, vp, cb);
     W v = cst[r];
-    const T *Xc = X + (size_t)c * in_stride;
-    for (int k = b; k < e; k++) v += (W)Xc[col[k]] * val[k];
+    const T *Xc = X + (size_t)c * in_stride + cb;
+    for (int k = b; k < e; k++) v += (W)Xc[cp[k]] * vp[k];
     W o = alpha * v;
     if (yoff) o = (W)X[yoff[r]] + o;
     dst[out[r] + (size_t)c * out_stride] = (T)o;
@@ -47,13 +82,16 @@ __global__ void __launch_bounds__(BLOCK) k_interp_l(int nrow, int ncomp, const i
                                                     const int *__restrict__ col, const W *__restrict__ val,
                                                     const W *__restrict__ cst, const int *__restrict__ out,
                                                     const int *__restrict__ yoff, W alpha, int in_stride,
-                                                    int out_stride, const T *X, T *dst) {
+                                                    int out_stride, const T *X, T *dst, GTempl<W> tp) {
     const long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) / L;
     const int lane = threadIdx.x % L;
     const bool in = g < (long)nrow * ncomp;
     const int r = in ? (int)(g % nrow) : 0, c = in ? (int)(g / nrow) : 0;
-    const int b = in ? rowptr[r] : 0, e = in ? rowptr[r + 1] : 0;
-    const T *Xc = X + (size_t)c * in_stride;
+    int b = 0, e = 0, cb = 0;
+    const int *cp = col;
+    const W *vp = val;
+    if (in) grow<W>(tp, r, rowptr, col, val, b, e, cp, vp, cb);
+    const T *Xc = X + (size_t)c * in_stride + cb;
     W v = 0;
     // four entries per pass: indices first, then the gathered values
     constexpr int U = 4;
@@ -66,8 +104,8 @@ __global__ void __launch_bounds__(BLOCK) k_interp_l(int nrow, int ncomp, const i
         for (int q = 0; q < U; q++) {
             const int k = k0 + q * L;
             ok[q] = k < e;
-            j[q] = ok[q] ? col[k] : 0;
-            a[q] = ok[q] ? val[k] : W(0);
+            j[q] = ok[q] ? cp[k] : 0;
+            a[q] = ok[q] ? vp[k] : W(0);
         }
 #pragma unroll
         for (int q = 0; q < U; q++) {
