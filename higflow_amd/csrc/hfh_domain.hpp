@@ -3,6 +3,7 @@
 // pdomain.c:1146-1221, with boundary values kept symbolic.
 #pragma once
 #include <array>
+#include <stdexcept>
 #include <atomic>
 #include <thread>
 
@@ -434,8 +435,12 @@ class Domain {
             int c = t->cell_with_point(at);
             if (c >= 0) return t->delta(c)[pd];
         }
-        std::fprintf(stderr, "hfh: boundary projection outside the domain\n");
-        std::abort();
+        // a Dirichlet boundary whose projection of a ghost point finds no
+        // domain cell: malformed boundary trees.  Set-up reports it through
+        // the entry point's error (hfh_last_error) instead of ending the
+        // caller's process.
+        throw std::runtime_error("hfh: boundary projection outside the domain (boundary tree does not border a "
+                                 "domain cell)");
     }
 
     double side_delta(Boundary<D> *b, int bcell, int pd) const {

@@ -92,3 +92,17 @@ def test_host_only_calls(host_only):
     assert h.probe(0, 1) is None  # no device: refused, not crashed
     with pytest.raises(native.HipError):
         h.step()
+
+
+def test_malformed_boundary_is_an_error_not_an_abort():
+    # a Dirichlet boundary tree reaching past the domain: a ghost point near
+    # the corner projects onto it outside every domain cell.  The reference
+    # asserts there (domain.c:939-940); the library reports it through the
+    # entry point and the caller's process goes on.
+    from higflow_amd import cases
+    c = cases.ve_cavity(8)
+    c["bcs"][1]["amr"] = cases.amr_boundary([1.0, 0.0], [1.0, 2.0], [1, 16])
+    with pytest.raises(native.HipError, match="boundary projection outside the domain"):
+        native.Flow(c, host_only=True)
+    # the library still works afterwards
+    native.Flow(cases.ve_cavity(8), host_only=True)
