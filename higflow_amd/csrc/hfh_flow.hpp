@@ -345,6 +345,8 @@ class Flow : public FlowBase {
     hipEvent_t ev0_[ST_NSTAGE] = {}, ev1_[ST_NSTAGE] = {};
     bool timing_on_ = false;
     bool verbose_ = false;  // HFH_VERBOSE=1: velocity range per step
+    DevBuf<double> tT_;     // polymeric stress (field 9), on request
+    const double *poly_stress(int comp);
     DevBuf<double> mm_;     // k_minmax output and workgroup partials
     DevBuf<unsigned> mmt_;  // k_minmax ticket
     double ms_[ST_NSTAGE] = {0};

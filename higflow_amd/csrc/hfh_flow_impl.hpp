@@ -654,6 +654,16 @@ int Flow<D>::step(bool ve) {
             k_constitutive<D><<<nblocks(ne), BLOCK, 0, st_>>>(vp, dVeSlots_.p, dVeFlags_.p, dVeDelta_.p, X);
         HFH_CHECK(hipMemcpyAsync(X + oK_, X + oKN_, sizeof(double) * NSYM * ne, hipMemcpyDeviceToDevice, st_));
         tend(ST_VE);
+        if (verbose_ && mem_.empty())
+            // the polymeric-tensor step's stress range (hig-flow-step-
+            // viscoelastic.c:326-335), lower triangle
+            for (int i = 0; i < D; i++)
+                for (int j = 0; j <= i; j++) {
+                    double mn, mx;
+                    field_range(9, i * D + j, &mn, &mx);
+                    std::printf("===> %d %d: Tmin = %15.10lf <===> Tmax = %15.10lf <===\n", i, j, mn, mx);
+                    std::fflush(stdout);
+                }
         // the viscoelastic step refreshes boundary values and sources after
         // the tensor updates (hig-flow-step-viscoelastic.c:2112-2125)
         boundary_refresh();
@@ -957,8 +967,19 @@ int Flow<D>::field_size(int field, int comp) {
     case 7:
     case 8:
         return comp >= 0 && comp < D * D ? ne : -1;
+    case 9:  // polymeric stress, computed on request
+        return par_.flowtype == VISCOELASTIC && comp >= 0 && comp < D * D ? ne : -1;
     }
     return -1;
+}
+
+// field 9 component `comp` into tT_ (T = S + (1 - beta) Du / Re)
+template <int D>
+const double *Flow<D>::poly_stress(int comp) {
+    if ((int)tT_.n < ne_) tT_.alloc(ne_);
+    k_poly_stress<<<nblocks(ne_), BLOCK, 0, st_>>>(ne_, X_.p + oS_ + comp * ne_, X_.p + oDv_ + comp * ne_, par_.beta,
+                                                   par_.Re, tT_.p);
+    return tT_.p;
 }
 
 template <int D>
@@ -1005,8 +1026,8 @@ template <int D>
 int Flow<D>::get_field(int field, int comp, double *out) {
     int n = field_size(field, comp);
     if (n < 0 || host_only_ || !mem_.empty()) return -1;
-    int off = field_offset(field, comp);
-    HFH_CHECK(hipMemcpyAsync(out, X_.p + off, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
+    const double *src = field == 9 ? poly_stress(comp) : X_.p + field_offset(field, comp);
+    HFH_CHECK(hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
     return n;
 }
@@ -1021,7 +1042,8 @@ int Flow<D>::field_range(int field, int comp, double *mn, double *mx) {
     if ((long)mm_.n < 2 + 2L * g) mm_.alloc(2 + 2 * (size_t)g);  // min, max, then the partials
     if (!mmt_.p) mmt_.alloc(1);
     k_zero_u32<<<1, 64, 0, st_>>>(mmt_.p);
-    k_minmax<<<g, 1024, 0, st_>>>(n, X_.p + field_offset(field, comp), mm_.p + 2, mmt_.p, mm_.p);
+    const double *src = field == 9 ? poly_stress(comp) : X_.p + field_offset(field, comp);
+    k_minmax<<<g, 1024, 0, st_>>>(n, src, mm_.p + 2, mmt_.p, mm_.p);
     double h[2];
     HFH_CHECK(hipMemcpyAsync(h, mm_.p, sizeof h, hipMemcpyDeviceToHost, st_));
     HFH_CHECK(hipStreamSynchronize(st_));
@@ -1033,7 +1055,7 @@ int Flow<D>::field_range(int field, int comp, double *mn, double *mx) {
 template <int D>
 int Flow<D>::set_field(int field, int comp, const double *in) {
     int n = field_size(field, comp);
-    if (n < 0 || host_only_ || !mem_.empty()) return -1;
+    if (n < 0 || host_only_ || !mem_.empty() || field == 9) return -1;
     if (field == 6 && comp / D > comp % D) return n;  // symmetric storage keeps the upper triangle
     int off = field_offset(field, comp);
     extrap_hist_ = 0;

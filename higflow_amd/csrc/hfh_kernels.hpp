@@ -430,6 +430,16 @@ __global__ void __launch_bounds__(1024) k_minmax(long n, const double *__restric
     }
 }
 
+// Polymeric stress of the reference's Tmin/Tmax log (hig-flow-step-
+// viscoelastic.c:319-324): T = S + (1 - beta) Du / Re, in its order of
+// operations
+__global__ void __launch_bounds__(BLOCK) k_poly_stress(int n, const double *__restrict__ S,
+                                                       const double *__restrict__ Du, double beta, double Re,
+                                                       double *__restrict__ T) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) T[i] = S[i] + (1.0 - beta) * Du[i] / Re;
+}
+
 __global__ void k_zero_u32(unsigned *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }

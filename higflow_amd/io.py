@@ -195,8 +195,8 @@ def print_vtk(sim, nameprint, rank=0, frame=0, Re=1.0, beta=0.0, viscoelastic=Fa
     ``fields`` maps (field id, component) to values already on the host;
     missing ones are read from ``sim``.  The stencils come from the
     library's host mesh (``hfh_stencil``).  Returns the file name."""
-    if sim.dim != 2:
-        raise NotImplementedError("print_vtk: the 2D writer only (higflow_print_vtk2D)")
+    if sim.dim == 3:
+        return print_vtk3d(sim, nameprint, rank, frame, viscoelastic, fields)
     dim = 2
     need = [(0, d) for d in range(dim)] + [(1, 0)]
     if viscoelastic:
@@ -239,6 +239,58 @@ def print_vtk(sim, nameprint, rank=0, frame=0, Re=1.0, beta=0.0, viscoelastic=Fa
     out.append("\nCELL_DATA %d\nSCALARS p FLOAT\nLOOKUP_TABLE default\n" % n)
     for k in range(n):
         out.append("%e\n" % _at(sim, f, 0, 0, 1, 0, cen[k], cen[k]))
+    with open(name, "w", encoding="utf-8") as fh:
+        fh.write("".join(out))
+    return name
+
+
+def print_vtk3d(sim, nameprint, rank=0, frame=0, viscoelastic=False, fields=None):
+    """Writes ``<nameprint>_<rank>-<frame>.vtk`` as the reference's 3D writers
+    do: higflow_print_vtk3D for Newtonian flows (hig-flow-io.c:2287-2791),
+    higflow_print_vtk3D_viscoelastic for viscoelastic ones (:2795-3033):
+    * the leaf cells as VTK hexahedra, eight corner points each (``%lf``);
+    * ``vel``: the velocity interpolated at each corner from the cell's
+      centre through the velocity-facet stencils;
+    * Newtonian: ``p`` at the cell centres; viscoelastic: the cell values of
+      S00 under a second CELL_DATA header (the pressure block of that
+      writer is commented out in the reference).
+
+    ``fields`` maps (field id, component) to values already on the host;
+    missing ones are read from ``sim``.  Returns the file name."""
+    dim = 3
+    need = [(0, d) for d in range(dim)] + [(7, 0) if viscoelastic else (1, 0)]
+    f = dict(fields or {})
+    for key in need:
+        if key not in f:
+            f[key] = sim.get(*key)
+    lo, hi = sim.cell_bounds()
+    cen = np.asarray(sim.centers(1, 0)).reshape(-1, dim)
+    n = len(cen)
+
+    def corners(k):
+        a, b = lo[k], hi[k]
+        return ((a[0], a[1], a[2]), (b[0], a[1], a[2]), (b[0], b[1], a[2]), (a[0], b[1], a[2]),
+                (a[0], a[1], b[2]), (b[0], a[1], b[2]), (b[0], b[1], b[2]), (a[0], b[1], b[2]))
+    name = "%s_%d-%d.vtk" % (nameprint, rank, frame)
+    out = ["# vtk DataFile Version 3.0\nhigtree\nASCII\nDATASET UNSTRUCTURED_GRID\n", "POINTS %d float\n" % (8 * n)]
+    for k in range(n):
+        out.append("".join("%f %f %f\n" % p for p in corners(k)))
+    out.append("CELLS %d %d\n" % (n, 9 * n))
+    out.extend("8 %s\n" % " ".join(str(8 * i + q) for q in range(8)) for i in range(n))
+    out.append("CELL_TYPES %d\n" % n)
+    out.append("12 " * n)
+    out.append("\nPOINT_DATA %d\nVECTORS vel FLOAT\n" % (8 * n))
+    for k in range(n):
+        c = cen[k]
+        for p in corners(k):
+            x = np.array(p)
+            out.append("%e %e %e\n" % tuple(_at(sim, f, 1, d, 0, d, c, x) for d in range(dim)))
+    if viscoelastic:
+        out.append("\nCELL_DATA %d\nSCALARS S00 FLOAT\nLOOKUP_TABLE default\n" % n)
+        out.extend("%e\n" % v for v in f[(7, 0)])
+    else:
+        out.append("\nCELL_DATA %d\nSCALARS p FLOAT\nLOOKUP_TABLE default\n" % n)
+        out.extend("%e\n" % v for v in f[(1, 0)])
     with open(name, "w", encoding="utf-8") as fh:
         fh.write("".join(out))
     return name

@@ -145,7 +145,10 @@ int hfh_set_functions(hfh_solver *s, const hfh_functions *f);
 int hfh_bc_set_valuetype(hfh_solver *s, int kind, int bcindex, int timedependent);
 
 /* fields: 0 u[comp], 1 p, 2 deltap, 3 ustar[comp], 4 F, 5 FU[comp],
- *         6 kernel[i*DIM+j], 7 S[i*DIM+j], 8 Du[i*DIM+j]
+ *         6 kernel[i*DIM+j], 7 S[i*DIM+j], 8 Du[i*DIM+j],
+ *         9 polymeric stress T[i*DIM+j] = S + (1 - beta) Du / Re (read only,
+ *           computed on request; the tensor of the reference's Tmin/Tmax
+ *           line, hig-flow-step-viscoelastic.c:319-335)
  * in the reference's local-id order (cells: depth-first leaves; facets:
  * the facet iterator of domain.c:1834-1894). */
 int hfh_field_size(hfh_solver *s, int field, int comp);
@@ -154,7 +157,9 @@ int hfh_get_field(hfh_solver *s, int field, int comp, double *out);
 int hfh_set_field(hfh_solver *s, int field, int comp, const double *in);
 /* min and max of a field on the device, from +-1e16 (for u[comp]: the
  * Vmin/Vmax that higflow_final_velocity prints, hig-flow-step.c:250-306);
- * returns the field size.  With HFH_VERBOSE=1 every step prints that line. */
+ * returns the field size.  With HFH_VERBOSE=1 every step prints that line,
+ * and a viscoelastic step first prints the Tmin/Tmax line of field 9 for
+ * each i >= j. */
 int hfh_field_range(hfh_solver *s, int field, int comp, double *vmin, double *vmax);
 /* No counterpart in the reference: hfh_field_range for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops

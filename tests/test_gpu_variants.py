@@ -126,3 +126,29 @@ def test_time_dependent_boundaries_and_sources(ve, proj):
     case = cases.newt_channel(nx=40, ny=10, projtype=proj, **kw, **TIGHT)
     w = compare(case, 4, init=_time_dependent, floor=ve)
     print(w)
+
+
+def test_polymeric_stress_range_log(monkeypatch, capfd):
+    # the Tmin/Tmax line of the polymeric-tensor step (hig-flow-step-
+    # viscoelastic.c:319-335) for i >= j, then the velocity line, each step
+    monkeypatch.setenv("HFH_VERBOSE", "1")
+    case = cases.ve_cavity(16, Re=1.0, Wi=1.0, beta=0.5, dt=2e-4, **TIGHT)
+    o, h = hfo.Oracle(case), native.Flow(case)
+    for _ in range(3):
+        o.step()
+        h.step()
+    out = capfd.readouterr().out.splitlines()
+    tl = [ln for ln in out if "Tmin" in ln]
+    assert len(tl) == 3 * 3
+    beta, Re = 0.5, 1.0
+    for k, (i, j) in enumerate(((0, 0), (1, 0), (1, 1))):
+        c = i * 2 + j
+        T = h.get(9, c)
+        # field 9 is S + (1 - beta) Du / Re in the reference's order
+        assert np.array_equal(T, h.get(7, c) + (1.0 - beta) * h.get(8, c) / Re)
+        To = o.get(7, c) + (1.0 - beta) * o.get(8, c) / Re
+        scale = np.abs(To).max()
+        mn, mx = h.range(9, c)
+        assert (mn, mx) == (T.min(), T.max())
+        assert abs(mn - To.min()) <= 1e-10 * scale and abs(mx - To.max()) <= 1e-10 * scale
+        assert tl[-3 + k] == "===> %d %d: Tmin = %15.10f <===> Tmax = %15.10f <===" % (i, j, mn, mx)

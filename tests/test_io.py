@@ -4,6 +4,7 @@ round-trip (run on the CPU oracle, which shares the field interface)."""
 import os
 
 import numpy as np
+import pytest
 
 import hfo
 from higflow_amd import cases, io
@@ -131,3 +132,53 @@ def test_vtk_writer_viscoelastic_tensor(tmp_path, monkeypatch):
     t = _vtk_section(lines, "TENSORS τₚ FLOAT", 12 * n).reshape(4 * n, 3, 3)
     want = S0 + 2.0 * (1 - 0.25) * 0.5 * D0 / 2.0
     assert np.abs(t[:, :2, :2] - want).max() < 1e-6 and not t[:, 2, :].any() and not t[:, :, 2].any()
+
+
+@pytest.mark.parametrize("viscoelastic", [False, True])
+def test_vtk_writer_3d(tmp_path, monkeypatch, viscoelastic):
+    # higflow_print_vtk3D / higflow_print_vtk3D_viscoelastic layout on a
+    # 3D wall-layer AMR cavity with a linear velocity field: every corner,
+    # coarse or fine, interpolates it exactly (quadratic WLS reproduces
+    # linear fields; walls hold the field's own boundary values)
+    monkeypatch.setenv("HFH_HOST_ONLY", "1")
+    from higflow_amd import native
+    G = np.array([[0.3, -0.2, 0.1], [0.05, 0.2, -0.4], [0.15, 0.1, -0.5]])
+    lin = lambda x, d: 0.1 * (d + 1) + G[d] @ np.asarray(x)
+    c = cases.wall_layer_cavity(4, dim=3, levels=1, layer=0.25)
+    if viscoelastic:
+        c["params"]["flowtype"] = cases.VISCOELASTIC
+    for bc in c["bcs"]:
+        bc["value"] = lambda kind, x: lin(x, kind - 1) if kind > 0 else 0.0
+    h = native.Flow(c)
+    n = h.size(1)
+    f = {(0, d): np.array([lin(x, d) for x in h.centers(0, d)]) for d in range(3)}
+    f[(1, 0)] = np.linspace(-1.0, 2.0, n)
+    if viscoelastic:
+        f[(7, 0)] = np.linspace(0.5, 0.75, n)
+    name = io.print_vtk(h, str(tmp_path / "c3"), rank=1, frame=2, viscoelastic=viscoelastic, fields=f)
+    assert name.endswith("c3_1-2.vtk")
+    lines = open(name, encoding="utf-8").read().split("\n")
+    assert lines[:5] == ["# vtk DataFile Version 3.0", "higtree", "ASCII", "DATASET UNSTRUCTURED_GRID",
+                         "POINTS %d float" % (8 * n)]
+    pts = _vtk_section(lines, "POINTS %d float" % (8 * n), 8 * n)
+    lo, hi = h.cell_bounds()
+    assert np.allclose(pts[0::8], lo, atol=1e-6) and np.allclose(pts[6::8], hi, atol=1e-6)
+    cells = _vtk_section(lines, "CELLS %d %d" % (n, 9 * n), n)
+    assert np.array_equal(cells[:, 0], np.full(n, 8)) and np.array_equal(cells[:, 1:].ravel(), np.arange(8 * n))
+    assert lines[lines.index("CELL_TYPES %d" % n) + 1] == "12 " * n
+    vel = _vtk_section(lines, "VECTORS vel FLOAT", 8 * n)
+    want = np.array([[lin(p, d) for d in range(3)] for p in pts])
+    # corners off the x and y walls: on those walls the boundary interpolant
+    # is evaluated at the reference's unreduced projection point
+    # (get_stencil_dirichlet_boundary, domain.c:1271-1280), not at the wall
+    # point itself
+    off = np.all((pts[:, :2] > 1e-9) & (pts[:, :2] < 1.0 - 1e-9), axis=1)
+    assert off.sum() > 8 * n // 2
+    assert np.abs(vel[off] - want[off]).max() < 1e-6
+    key = "SCALARS S00 FLOAT" if viscoelastic else "SCALARS p FLOAT"
+    i = lines.index(key)
+    assert lines[i - 1] == "CELL_DATA %d" % n and lines[i + 1] == "LOOKUP_TABLE default"
+    vals = np.array([float(v) for v in lines[i + 2:i + 2 + n]])
+    src = f[(7, 0)] if viscoelastic else f[(1, 0)]
+    assert np.array_equal(vals, np.array(["%e" % v for v in src], dtype=float))
+    assert ("SCALARS p FLOAT" in lines) != viscoelastic
