@@ -22,6 +22,7 @@ template <typename T>
 class FusedKrylov {
   public:
     int n = 0, nb = 0, lanes = 1;
+    int rows = 1;  // rows per lane group (R of the Krylov kernels)
     const int *rp = nullptr, *ci = nullptr;
     KBuf<T> val, vhat, dinv;  // A, A D^-1, D^-1
     KBuf<T> r, rh, z, p, p2, q, v, v2, s, t, part, ks;
@@ -83,7 +84,6 @@ class FusedKrylov {
         ks.alloc((size_t)KS_N * nseg);
         ki.alloc((size_t)KI_N * nseg);
         pred = 0;
-        S = KSeg();
         tune(st);
         // defined contents before the first solve (a cached lane choice skips
         // the timing launches that would otherwise have written them)
@@ -91,22 +91,6 @@ class FusedKrylov {
         k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
-        if (nseg > 1) {
-            const int rpw = KNT / lanes;
-            std::vector<int> ws(nb), wb(nseg + 1);
-            for (int q = 0; q <= nseg; q++) wb[q] = seg_off[q] / rpw;
-            for (int q = 0; q < nseg; q++)
-                for (int w = wb[q]; w < wb[q + 1]; w++) ws[w] = q;
-            d_wseg.alloc(nb);
-            d_wbeg.alloc(nseg + 1);
-            if (hipMemcpyAsync(d_wseg.p, ws.data(), nb * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(d_wbeg.p, wb.data(), (nseg + 1) * sizeof(int), hipMemcpyHostToDevice, st) !=
-                    hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                throw std::runtime_error("segment upload failed");
-            S.wseg = d_wseg.p;
-            S.wbeg = d_wbeg.p;
-        }
         amg.ready = false;
         if (want_amg && amg.build(n, hrp, hci, hv, st)) {
             ph.alloc(n);
@@ -114,6 +98,43 @@ class FusedKrylov {
         }
     }
     bool amg_on() const { return amg.ready; }
+
+    // workgroup -> segment tables for L lanes per row (workgroups of
+    // KNT / L rows; segment offsets are multiples of KNT rows)
+    void set_segments(int L, hipStream_t st) {
+        S = KSeg();
+        if (nseg <= 1) return;
+        const int rpw = KNT / L;
+        std::vector<int> ws(nb), wb(nseg + 1);
+        for (int q = 0; q <= nseg; q++) wb[q] = seg_off[q] / rpw;
+        for (int q = 0; q < nseg; q++)
+            for (int w = wb[q]; w < wb[q + 1]; w++) ws[w] = q;
+        d_wseg.alloc(nb);
+        d_wbeg.alloc(nseg + 1);
+        if (hipMemcpyAsync(d_wseg.p, ws.data(), nb * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_wbeg.p, wb.data(), (nseg + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            throw std::runtime_error("segment upload failed");
+        S.wseg = d_wseg.p;
+        S.wbeg = d_wbeg.p;
+    }
+
+    // calls f(integral_constant L, integral_constant R) for the current
+    // lanes per row and rows per lane group
+    template <typename F>
+    void with_shape(F &&f) const {
+        using std::integral_constant;
+        switch (lanes * 10 + rows) {
+        case 11: f(integral_constant<int, 1>{}, integral_constant<int, 1>{}); break;
+        case 12: f(integral_constant<int, 1>{}, integral_constant<int, 2>{}); break;
+        case 14: f(integral_constant<int, 1>{}, integral_constant<int, 4>{}); break;
+        case 21: f(integral_constant<int, 2>{}, integral_constant<int, 1>{}); break;
+        case 22: f(integral_constant<int, 2>{}, integral_constant<int, 2>{}); break;
+        case 24: f(integral_constant<int, 2>{}, integral_constant<int, 4>{}); break;
+        case 41: f(integral_constant<int, 4>{}, integral_constant<int, 1>{}); break;
+        default: f(integral_constant<int, 8>{}, integral_constant<int, 1>{}); break;
+        }
+    }
 
     // Rows whose (column offset, weight of A, weight of A D^-1) lists
     // repeat become template rows: a list is kept as a template when at
@@ -235,22 +256,14 @@ class FusedKrylov {
     template <typename TB>
     void init(bool cg, const TB *b, T *x, hipStream_t st, bool xzero = false) {
         cg = cg && !amg.ready;
-        switch (lanes) {
-        case 1: launch_init<TB, 1>(cg, xzero, b, x, st); break;
-        case 2: launch_init<TB, 2>(cg, xzero, b, x, st); break;
-        case 4: launch_init<TB, 4>(cg, xzero, b, x, st); break;
-        default: launch_init<TB, 8>(cg, xzero, b, x, st); break;
-        }
+        with_shape([&](auto Lc, auto Rc) { launch_init<TB, Lc.value, Rc.value>(cg, xzero, b, x, st); });
     }
     // iterations [it0, it0 + count)
     void iterate(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
         cg = cg && !amg.ready;
-        switch (lanes) {
-        case 1: launch_iter<1>(cg, it0, count, maxit, rtol, atol, x, st); break;
-        case 2: launch_iter<2>(cg, it0, count, maxit, rtol, atol, x, st); break;
-        case 4: launch_iter<4>(cg, it0, count, maxit, rtol, atol, x, st); break;
-        default: launch_iter<8>(cg, it0, count, maxit, rtol, atol, x, st); break;
-        }
+        with_shape([&](auto Lc, auto Rc) {
+            launch_iter<Lc.value, Rc.value>(cg, it0, count, maxit, rtol, atol, x, st);
+        });
     }
 
     // Times `reps` launches of BiCGStab launch B, t = (A D^-1)(r - alpha v)
@@ -260,14 +273,9 @@ class FusedKrylov {
     // ids, or CSR structure and values), r, v and rh read once, s and t
     // written once, the partials.
     float probe(int reps, long *bytes, hipStream_t st) {
-        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
+        k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
         float ms = 0;
-        switch (lanes) {
-        case 1: ms = time_b<1>(reps, st); break;
-        case 2: ms = time_b<2>(reps, st); break;
-        case 4: ms = time_b<4>(reps, st); break;
-        default: ms = time_b<8>(reps, st); break;
-        }
+        with_shape([&](auto Lc, auto Rc) { ms = time_b<Lc.value, Rc.value>(reps, st, S); });
         *bytes = op_bytes_ + (long)n * 5 * sizeof(T) + (long)nb * 6 * sizeof(T);
         return ms;
     }
@@ -283,28 +291,24 @@ class FusedKrylov {
     // p, v, x, D^-1 and rh read once; x, r, p_new and v_new written once;
     // the partials.
     float probe_a(int reps, long *bytes, hipStream_t st) {
-        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
-        const T slotv[4] = {T(1), T(1), T(2), T(1)};  // t.s, t.t, rh.s, rh.t
+        k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
+        // every partial of t.s, t.t, rh.s, rh.t set: omega = 1, rho > 0
+        const T slotv[4] = {T(1), T(1), T(2), T(1)};
         const int slot[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
         for (int k = 0; k < 4; k++)
-            k_fill<T><<<nblocks(nb), BLOCK, 0, st>>>(nb, part.p + (size_t)slot[k] * nb, T(0));
-        std::vector<T> hk(KS_N, T(0));
-        hk[KS_RHO + 0] = T(1);
-        hk[KS_ALPHA + 0] = T(0.5);
-        if (hipStreamSynchronize(st) != hipSuccess ||
-            hipMemcpy(ks.p, hk.data(), KS_N * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-            throw std::runtime_error("probe set-up failed");
-        for (int k = 0; k < 4; k++)
-            if (hipMemcpy(part.p + (size_t)slot[k] * nb, &slotv[k], sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-                throw std::runtime_error("probe set-up failed");
-        float ms = 0;
-        switch (lanes) {
-        case 1: ms = time_a<1>(reps, st); break;
-        case 2: ms = time_a<2>(reps, st); break;
-        case 4: ms = time_a<4>(reps, st); break;
-        default: ms = time_a<8>(reps, st); break;
+            k_fill<T><<<nblocks(nb), BLOCK, 0, st>>>(nb, part.p + (size_t)slot[k] * nb, slotv[k]);
+        std::vector<T> hk((size_t)KS_N * nseg, T(0));
+        for (int g = 0; g < nseg; g++) {
+            hk[(size_t)g * KS_N + KS_RHO] = T(1);
+            hk[(size_t)g * KS_N + KS_ALPHA] = T(0.5);
         }
+        if (hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(ks.p, hk.data(), hk.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("probe set-up failed");
+        float ms = 0;
+        with_shape([&](auto Lc, auto Rc) { ms = time_a<Lc.value, Rc.value>(reps, st, S); });
         *bytes = op_bytes_ + (long)n * 11 * sizeof(T) + (long)nb * 6 * sizeof(T);
+        k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         return ms;
     }
 
@@ -329,12 +333,7 @@ class FusedKrylov {
             hipMemcpy(ks.p, hk.data(), hk.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
             throw std::runtime_error("probe set-up failed");
         float ms = 0;
-        switch (lanes) {
-        case 1: ms = time_cg<1>(reps, st); break;
-        case 2: ms = time_cg<2>(reps, st); break;
-        case 4: ms = time_cg<4>(reps, st); break;
-        default: ms = time_cg<8>(reps, st); break;
-        }
+        with_shape([&](auto Lc, auto Rc) { ms = time_cg<Lc.value, Rc.value>(reps, st); });
         *bytes = op_bytes_ + (long)n * 4 * sizeof(T) + (long)nb * 4 * sizeof(T);
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
@@ -342,15 +341,15 @@ class FusedKrylov {
     }
 
   private:
-    template <int L>
+    template <int L, int R>
     float time_cg(int reps, hipStream_t st) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
-            k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), Av, z.p, p.p, p2.p, q.p,
-                                                 part.p, nb, ks.p, ki.p, S);
+            k_kcg_spmv<T, L, R><<<nb, KNT / R, 0, st>>>(n, 1, 1 << 30, T(0), T(0), Av, z.p, p.p, p2.p, q.p,
+                                                        part.p, nb, ks.p, ki.p, S);
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
         float ms = 0;
@@ -359,15 +358,15 @@ class FusedKrylov {
         (void)hipEventDestroy(b);
         return reps > 0 ? ms / reps : 0.0f;
     }
-    template <int L>
-    float time_a(int reps, hipStream_t st) {
+    template <int L, int R>
+    float time_a(int reps, hipStream_t st, KSeg sg) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
-            k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, 1, 1 << 30, T(0), T(0), Ah, dinv.p, q.p, r.p, rh.p, s.p,
-                                               t.p, p.p, v.p, p2.p, v2.p, part.p, nb, ks.p, ki.p, KSeg());
+            k_kbcg_a<T, L, R><<<nb, KNT / R, 0, st>>>(n, 1, 1 << 30, T(0), T(0), Ah, dinv.p, q.p, r.p, rh.p,
+                                                      s.p, t.p, p.p, v.p, p2.p, v2.p, part.p, nb, ks.p, ki.p, sg);
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
         float ms = 0;
@@ -393,13 +392,13 @@ class FusedKrylov {
         return std::max(1, (n + KNT / L - 1) / (KNT / L));
     }
 
-    template <typename TB, int L>
+    template <typename TB, int L, int R>
     void launch_init(bool cg, bool xzero, const TB *b, T *x, hipStream_t st) {
-        k_kinit<T, TB, L><<<nb, KNT, 0, st>>>(n, cg ? 1 : 0, xzero ? 1 : 0, Av, x, b, dinv.p, r.p, z.p,
-                                              rh.p, part.p, nb,
-                                              ki.p, S);
+        k_kinit<T, TB, L, R><<<nb, KNT / R, 0, st>>>(n, cg ? 1 : 0, xzero ? 1 : 0, Av, x, b, dinv.p, r.p, z.p,
+                                                     rh.p, part.p, nb, ki.p, S);
     }
-    template <int L>
+    // the V-cycle path keeps R = 1 (its launches are bound by the cycle)
+    template <int L, int R>
     void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
         if (amg.ready) {
             // one segment: the cycle stops with the solve; several: it runs
@@ -420,28 +419,29 @@ class FusedKrylov {
         for (int it = it0; it < it0 + count; it++) {
             T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
             if (cg) {
-                k_kcg_spmv<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, Av, z.p, pold, pnew, q.p,
-                                                     part.p, nb, ks.p, ki.p, S);
-                k_kcg_update<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb, ks.p, ki.p,
-                                                       S);
+                k_kcg_spmv<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, maxit, rtol, atol, Av, z.p, pold, pnew, q.p,
+                                                            part.p, nb, ks.p, ki.p, S);
+                k_kcg_update<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb,
+                                                              ks.p, ki.p, S);
             } else {
                 T *vold = (it & 1) ? v.p : v2.p, *vnew = (it & 1) ? v2.p : v.p;
-                k_kbcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, Ah, dinv.p, x, r.p, rh.p, s.p,
-                                                   t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p, S);
-                k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, Ah, r.p, rh.p, vnew, s.p, t.p, part.p,
-                                                   nb, ks.p, ki.p, S);
+                k_kbcg_a<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, maxit, rtol, atol, Ah, dinv.p, x, r.p, rh.p,
+                                                          s.p, t.p, pold, vold, pnew, vnew, part.p, nb, ks.p, ki.p,
+                                                          S);
+                k_kbcg_b<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, maxit, Ah, r.p, rh.p, vnew, s.p, t.p, part.p,
+                                                          nb, ks.p, ki.p, S);
             }
         }
     }
-    template <int L>
-    float time_b(int reps, hipStream_t st) {
+    template <int L, int R>
+    float time_b(int reps, hipStream_t st, KSeg sg) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < reps; rep++)
-            k_kbcg_b<T, L><<<nb, KNT, 0, st>>>(n, 0, 1 << 30, Ah, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
-                                               ks.p, ki.p, KSeg());
+            k_kbcg_b<T, L, R><<<nb, KNT / R, 0, st>>>(n, 0, 1 << 30, Ah, r.p, rh.p, v.p, s.p, t.p, part.p, nb,
+                                                      ks.p, ki.p, sg);
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
         float ms = 0;
@@ -450,43 +450,49 @@ class FusedKrylov {
         (void)hipEventDestroy(b);
         return reps > 0 ? ms / reps : 0.0f;
     }
-    // one BiCGStab iteration's two launches (A and B) with L lanes per row,
-    // on zero vectors with the flags cleared and launch A's scalars set so
-    // that it runs its update and SpMV (as probe_a): reads and writes only
-    // buffers of this solver.  Best of two timings of 25 launches each.
-    template <int L>
-    float time_lanes(hipStream_t st) {
+    // one BiCGStab iteration's two launches (A and B) with L lanes per row
+    // and R rows per lane group, on zero vectors with the flags cleared and
+    // launch A's scalars set so that every segment runs its update and
+    // SpMV (as probe_a): reads and writes only buffers of this solver.
+    // Best of two timings of 25 launches each.
+    template <int L, int R>
+    float time_shape(hipStream_t st) {
         lanes = L;
+        rows = R;
         nb = blocks<L>();
         part.alloc((size_t)KP_N * nb);
+        set_segments(L, st);
         for (KBuf<T> *b : {&r, &rh, &v, &v2, &s, &t, &p, &p2, &q}) k_fill<T><<<nblocks(n), BLOCK, 0, st>>>(n, b->p, T(0));
         k_fill<T><<<nblocks((long)KP_N * nb), BLOCK, 0, st>>>(KP_N * nb, part.p, T(0));
-        k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
-        k_fill<int><<<1, BLOCK, 0, st>>>(KI_N, ki.p, 0);
-        time_b<L>(2, st);
-        const float tb = std::min(time_b<L>(25, st), time_b<L>(25, st));
-        const T slotv[4] = {T(1), T(1), T(2), T(1)};  // t.s, t.t, rh.s, rh.t
+        k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
+        k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
+        time_b<L, R>(2, st, S);
+        const float tb = std::min(time_b<L, R>(25, st, S), time_b<L, R>(25, st, S));
+        // every partial of t.s, t.t, rh.s, rh.t set: omega = 1, rho > 0
+        const T slotv[4] = {T(1), T(1), T(2), T(1)};
         const int slot[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
-        std::vector<T> hk(KS_N, T(0));
-        hk[KS_RHO + 0] = T(1);
-        hk[KS_ALPHA + 0] = T(0.5);
-        if (hipStreamSynchronize(st) != hipSuccess ||
-            hipMemcpy(ks.p, hk.data(), KS_N * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-            throw std::runtime_error("lane timing set-up failed");
         for (int k = 0; k < 4; k++)
-            if (hipMemcpy(part.p + (size_t)slot[k] * nb, &slotv[k], sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-                throw std::runtime_error("lane timing set-up failed");
-        time_a<L>(2, st);
-        const float ta = std::min(time_a<L>(25, st), time_a<L>(25, st));
-        k_fill<T><<<1, BLOCK, 0, st>>>(KS_N, ks.p, T(0));
+            k_fill<T><<<nblocks(nb), BLOCK, 0, st>>>(nb, part.p + (size_t)slot[k] * nb, slotv[k]);
+        std::vector<T> hk((size_t)KS_N * nseg, T(0));
+        for (int g = 0; g < nseg; g++) {
+            hk[(size_t)g * KS_N + KS_RHO] = T(1);
+            hk[(size_t)g * KS_N + KS_ALPHA] = T(0.5);
+        }
+        if (hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(ks.p, hk.data(), hk.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("lane timing set-up failed");
+        time_a<L, R>(2, st, S);
+        const float ta = std::min(time_a<L, R>(25, st, S), time_a<L, R>(25, st, S));
+        k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         return ta + tb;
     }
-    // picks the lanes per row with the fastest iteration (launches A and B
-    // of BiCGStab).  The choice is
+    // picks the lanes per row L and rows per lane group R with the fastest
+    // iteration (launches A and B of BiCGStab, with the system's segments).
+    // The row sums and partial sums depend on L, not on R.  The choice is
     // kept per matrix shape (rows, entries, segments, precision) for the
     // life of the process, so solvers built later for the same system (a
     // second run of the same case) sum in the same order and give the same
-    // iterates.
+    // iterates.  HFH_KRYLOV_LANES / HFH_KRYLOV_ROWS override the choice.
     void tune(hipStream_t st) {
         static std::mutex mu;
         static std::map<std::tuple<int, long, int, int>, int> chosen;
@@ -496,7 +502,7 @@ class FusedKrylov {
             if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nz = last;
         }
         const auto key = std::make_tuple(n, nz, nseg, (int)sizeof(T));
-        int best = 1;
+        int best = 11;
         bool known = false;
         {
             std::lock_guard<std::mutex> g(mu);
@@ -507,24 +513,39 @@ class FusedKrylov {
             }
         }
         if (!known) {
-            const float tl[4] = {time_lanes<1>(st), time_lanes<2>(st), time_lanes<4>(st), time_lanes<8>(st)};
-            const int ll[4] = {1, 2, 4, 8};
+            const int shapes[8] = {11, 12, 14, 21, 22, 24, 41, 81};
             float tb = 1e30f;
-            for (int k = 0; k < 4; k++)
-                if (tl[k] < tb) {
-                    tb = tl[k];
-                    best = ll[k];
+            for (int sh : shapes) {
+                lanes = sh / 10;
+                rows = sh % 10;
+                float tm = 0;
+                with_shape([&](auto Lc, auto Rc) { tm = time_shape<Lc.value, Rc.value>(st); });
+                if (tm < tb) {
+                    tb = tm;
+                    best = sh;
                 }
+            }
             std::lock_guard<std::mutex> g(mu);
             chosen[key] = best;
         }
+        lanes = best / 10;
+        rows = best % 10;
         if (const char *e = std::getenv("HFH_KRYLOV_LANES")) {
             int f = std::atoi(e);
-            if (f == 1 || f == 2 || f == 4 || f == 8) best = f;
+            if (f == 1 || f == 2 || f == 4 || f == 8) lanes = f;
         }
-        lanes = best;
-        nb = best == 1 ? blocks<1>() : best == 2 ? blocks<2>() : best == 4 ? blocks<4>() : blocks<8>();
+        if (const char *e = std::getenv("HFH_KRYLOV_ROWS")) {
+            int f = std::atoi(e);
+            if (f == 1 || f == 2 || f == 4) rows = f;
+        }
+        if (lanes > 2) rows = 1;
+        nb = lanes == 1 ? blocks<1>() : lanes == 2 ? blocks<2>() : lanes == 4 ? blocks<4>() : blocks<8>();
         part.alloc((size_t)KP_N * nb);
+        set_segments(lanes, st);
+        if (const char *ev = std::getenv("HFH_VERBOSE"))
+            if (std::strcmp(ev, "1") == 0)
+                std::fprintf(stderr, "hfh: %zu-byte system of %d rows: %d lanes per row, %d rows per lane group\n",
+                             sizeof(T), n, lanes, rows);
     }
 };
 

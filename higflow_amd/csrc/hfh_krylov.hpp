@@ -17,7 +17,12 @@
 //   s = r - alpha v and t = (A D^-1) s.  The stopping test runs in launch B
 //   on the exact r.r of the residual launch A wrote.
 // * Each row is processed by L lanes (L = 1, 2, 4, 8, picked per matrix by
-//   timing at set-up); workgroups of 1024 threads hold 1024/L rows.
+//   timing at set-up); a workgroup holds 1024/L rows.  With R rows per lane
+//   group (R = 1, 2, 4) the workgroup has 1024/R threads and each lane
+//   group gathers R rows at once, which multiplies the loads in flight per
+//   compute unit when the row count, not the bytes, bounds the launch.
+//   The rows of a workgroup, the per-lane entry order and the partial-sum
+//   tree are the same for every R, so the iterates do not depend on R.
 //
 // Stopping: ||r|| <= max(rtol ||b||, atol) on the recursively updated
 // residual, a non-finite residual, breakdown, divergence, or `maxit`
@@ -136,19 +141,26 @@ __device__ inline void reduce_partials(const T *part, int nb, const KCtx &c, con
 
 __device__ inline void kblock_sync() { __syncthreads(); }
 
-// Sums K per-thread values over the workgroup and writes the totals to
-// part[slot[k] * nb + blockIdx.x] (fixed order: wave tree, then the wave
-// sums in sequence by thread 0).  Last shared-memory use of a launch.
-template <typename T, int K>
-__device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&slot)[K]) {
+// Sums K values over the workgroup's rows and writes the totals to
+// part[slot[k] * nb + blockIdx.x].  The workgroup has KNT/R threads; v[s]
+// holds this thread's values for slab s, the rows a workgroup of KNT
+// threads would give to its threads s KNT/R .. (s+1) KNT/R - 1.  Fixed
+// order, the same for every R: a tree within each wave of 64 of those
+// threads, then the KNW wave sums in sequence by one thread.  Last
+// shared-memory use of a launch.
+template <typename T, int K, int R = 1>
+__device__ inline void write_partials(T (&v)[R][K], T *part, int nb, const int (&slot)[K]) {
+    constexpr int NW = KNW / R;  // waves of this workgroup
     __shared__ T sw[K][KNW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        T a = v[k];
-        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-        if (lane == 0) sw[k][w] = a;
-    }
+    for (int sl = 0; sl < R; sl++)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            T a = v[sl][k];
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+            if (lane == 0) sw[k][sl * NW + w] = a;
+        }
     __syncthreads();
     if (threadIdx.x < K) {
         const int k = threadIdx.x;
@@ -156,6 +168,13 @@ __device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&sl
         for (int j = 0; j < KNW; j++) a += sw[k][j];
         part[(size_t)slot[k] * nb + blockIdx.x] = a;
     }
+}
+template <typename T, int K>
+__device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&slot)[K]) {
+    T w[1][K];
+#pragma unroll
+    for (int k = 0; k < K; k++) w[0][k] = v[k];
+    write_partials<T, K, 1>(w, part, nb, slot);
 }
 
 // stopping test; `first` computes the tolerance from ||b||.  Besides
@@ -198,6 +217,12 @@ template <int L>
 __device__ inline int krow() {
     return blockIdx.x * (KNT / L) + (int)threadIdx.x / L;
 }
+// row of this thread's slab s in a workgroup of KNT/R threads: the row
+// thread s KNT/R + threadIdx.x of a KNT-thread workgroup would take
+template <int L, int R>
+__device__ inline int krow_s(int s) {
+    return blockIdx.x * (KNT / L) + (s * (KNT / R) + (int)threadIdx.x) / L;
+}
 template <int L>
 __device__ inline int klane() {
     return (int)threadIdx.x % L;
@@ -219,100 +244,138 @@ struct KRows {
     const T *tw = nullptr;
 };
 
-// sums over row i's entries of val[k] * vec[m][col[k]], split over L lanes
-// and combined so that every lane holds the row totals (fixed order: lane l
-// takes entries l, l + L, ...; then a butterfly).  Each lane takes its
-// entries four at a time: the four column indices are loaded together,
-// then all their vector values, so a long (WLS) row costs a quarter of the
-// dependent load round trips.  A template row sums the same entries in the
-// same order as its CSR form.
-template <typename T, int L, int M>
-__device__ inline void row_sums(const KRows<T> &A, int i, bool in, const T *const (&vec)[M], T (&out)[M]) {
-    constexpr int U = 4;
+// sums over row i[s]'s entries of val[k] * vec[m][col[k]] for the R rows
+// of this lane group, split over L lanes and combined so that every lane
+// holds the row totals (fixed order: lane l takes entries l, l + L, ... of
+// each row in sequence; then a butterfly).  The entries are taken U at a
+// time for all R rows together: their column indices are loaded, then all
+// their vector values, so a lane group waits for one dependent round trip
+// per U entries of all its rows.  A template row sums the same entries in
+// the same order as its CSR form, and the order does not depend on R or U.
+template <typename T, int L, int M, int R>
+__device__ inline void row_sums_r(const KRows<T> &A, const int (&i)[R], const bool (&in)[R],
+                                  const T *const (&vec)[M], T (&out)[R][M]) {
+    constexpr int U = R >= 4 ? 2 : 4;
+    int rb[R], re[R], base[R], t[R];
+    const int *cp[R];
+    const T *vp[R];
 #pragma unroll
-    for (int m = 0; m < M; m++) out[m] = 0;
-    const int t = (in && A.code) ? A.code[i] : -1;
-    int rb = 0, re = 0;
-    const int *cp;
-    const T *vp;
-    int base;
-    if (t >= 0) {
-        rb = t * KT_MAX;
-        re = rb + A.tcnt[t];
-        cp = A.toff;
-        vp = A.tw;
-        base = i;
-    } else {
-        rb = in ? A.rp[i] : 0;
-        re = in ? A.rp[i + 1] : 0;
-        cp = A.ci;
-        vp = A.val;
-        base = 0;
+    for (int r = 0; r < R; r++) t[r] = (in[r] && A.code) ? A.code[i[r]] : -1;
+    int len = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int m = 0; m < M; m++) out[r][m] = 0;
+        if (t[r] >= 0) {
+            rb[r] = t[r] * KT_MAX;
+            re[r] = rb[r] + A.tcnt[t[r]];
+            cp[r] = A.toff;
+            vp[r] = A.tw;
+            base[r] = i[r];
+        } else {
+            rb[r] = in[r] ? A.rp[i[r]] : 0;
+            re[r] = in[r] ? A.rp[i[r] + 1] : 0;
+            cp[r] = A.ci;
+            vp[r] = A.val;
+            base[r] = 0;
+        }
+        len = max(len, re[r] - rb[r]);
     }
 #pragma unroll 1
-    for (int k0 = rb + klane<L>(); k0 < re; k0 += U * L) {
-        int j[U];
-        T a[U];
-        bool ok[U];
+    for (int k0 = klane<L>(); k0 < len; k0 += U * L) {
+        int j[R][U];
+        T a[R][U];
+        bool ok[R][U];
 #pragma unroll
-        for (int q = 0; q < U; q++) {
-            const int k = k0 + q * L;
-            ok[q] = k < re;
-            j[q] = ok[q] ? base + cp[k] : 0;
-            a[q] = ok[q] ? vp[k] : T(0);
-        }
+        for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int q = 0; q < U; q++)
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                const T x = vec[m][j[q]];
-                if (ok[q]) out[m] += a[q] * x;
+            for (int q = 0; q < U; q++) {
+                const int k = rb[r] + k0 + q * L;
+                ok[r][q] = k < re[r];
+                j[r][q] = ok[r][q] ? base[r] + cp[r][k] : 0;
+                a[r][q] = ok[r][q] ? vp[r][k] : T(0);
             }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < U; q++)
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const T x = vec[m][j[r][q]];
+                    if (ok[r][q]) out[r][m] += a[r][q] * x;
+                }
     }
 #pragma unroll
-    for (int m = 0; m < M; m++)
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) out[m] += __shfl_xor(out[m], o, 64);
+        for (int m = 0; m < M; m++)
+#pragma unroll
+            for (int o = L / 2; o > 0; o >>= 1) out[r][m] += __shfl_xor(out[r][m], o, 64);
+}
+template <typename T, int L, int M>
+__device__ inline void row_sums(const KRows<T> &A, int i, bool in, const T *const (&vec)[M], T (&out)[M]) {
+    const int ii[1] = {i};
+    const bool inn[1] = {in};
+    T o[1][M];
+    row_sums_r<T, L, M, 1>(A, ii, inn, vec, o);
+#pragma unroll
+    for (int m = 0; m < M; m++) out[m] = o[0][m];
 }
 
 // ------------------------------------------------------------------ init
 // r = b - A x; CG: z = dinv r, partial r.z; BiCGStab: rh = r, partial r.r.
 // Partials r.r and b.b; clears the flags.  xzero: x is set to zero here and
 // r = b (no product; the corrections of a refinement start from zero).
-template <typename T, typename TB, int L>
-__global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, KRows<T> A,
-                                               T *__restrict__ x, const TB *__restrict__ b,
-                                               const T *__restrict__ dinv, T *__restrict__ r, T *__restrict__ z,
-                                               T *__restrict__ rh, T *__restrict__ part, int nb, int *ki, KSeg S) {
+// Workgroups of KNT/R threads, R rows per lane group (slabs, see krow_s).
+template <typename T, typename TB, int L, int R = 1>
+__global__ void __launch_bounds__(KNT / R) k_kinit(int n, int cg, int xzero, KRows<T> A,
+                                                   T *__restrict__ x, const TB *__restrict__ b,
+                                                   const T *__restrict__ dinv, T *__restrict__ r, T *__restrict__ z,
+                                                   T *__restrict__ rh, T *__restrict__ part, int nb, int *ki, KSeg S) {
     T *ksd = nullptr;
     const KCtx c = kctx<T>(S, nb, ksd, ki);
-    const int i = krow<L>();
-    const bool in = i < n;
-    T sx[1] = {0};
+    int i[R];
+    bool in[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        i[s] = krow_s<L, R>(s);
+        in[s] = i[s] < n;
+    }
+    const bool l0 = klane<L>() == 0;
+    T sx[R][1];
+#pragma unroll
+    for (int s = 0; s < R; s++) sx[s][0] = 0;
     if (!xzero) {  // uniform over the launch
         const T *const vx[1] = {x};
-        row_sums<T, L, 1>(A, i, in, vx, sx);
-    } else if (in && klane<L>() == 0) {
-        x[i] = T(0);
+        row_sums_r<T, L, 1, R>(A, i, in, vx, sx);
+    } else {
+#pragma unroll
+        for (int s = 0; s < R; s++)
+            if (in[s] && l0) x[i[s]] = T(0);
     }
-    T acc[3] = {0, 0, 0};
-    if (in && klane<L>() == 0) {
-        T bi = (T)b[i];
-        T ri = bi - sx[0];
-        r[i] = ri;
-        if (cg) {
-            T zi = dinv[i] * ri;
-            z[i] = zi;
-            acc[0] = ri * zi;
-        } else {
-            rh[i] = ri;
-            acc[0] = ri * ri;
+    T acc[R][3];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        acc[s][0] = acc[s][1] = acc[s][2] = 0;
+        if (in[s] && l0) {
+            const int ii = i[s];
+            T bi = (T)b[ii];
+            T ri = bi - sx[s][0];
+            r[ii] = ri;
+            if (cg) {
+                T zi = dinv[ii] * ri;
+                z[ii] = zi;
+                acc[s][0] = ri * zi;
+            } else {
+                rh[ii] = ri;
+                acc[s][0] = ri * ri;
+            }
+            acc[s][1] = ri * ri;
+            acc[s][2] = bi * bi;
         }
-        acc[1] = ri * ri;
-        acc[2] = bi * bi;
     }
     const int sl[3] = {KP_RZ, KP_RR, KP_BB};
-    write_partials<T, 3>(acc, part, nb, sl);
+    write_partials<T, 3, R>(acc, part, nb, sl);
     if (c.lead && threadIdx.x == 0) {
         ki[KI_DONE] = 0;
         ki[KI_ITERS] = 0;
@@ -325,22 +388,32 @@ __global__ void __launch_bounds__(KNT) k_kinit(int n, int cg, int xzero, KRows<T
 // ------------------------------------------------------------------- PCG
 // launch 1: stopping test, beta = rho / rho_old, p = z + beta p_old (own rows
 // to pnew), q = A z + beta A p_old; partial p.q
-template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rtol, T atol,
-                                                  KRows<T> A, const T *__restrict__ z,
-                                                  const T *__restrict__ pold, T *__restrict__ pnew,
-                                                  T *__restrict__ q, T *__restrict__ part, int nb, T *ks, int *ki,
-                                                  KSeg S) {
+template <typename T, int L, int R = 1>
+__global__ void __launch_bounds__(KNT / R) k_kcg_spmv(int n, int it, int maxit, T rtol, T atol,
+                                                      KRows<T> A, const T *__restrict__ z,
+                                                      const T *__restrict__ pold, T *__restrict__ pnew,
+                                                      T *__restrict__ q, T *__restrict__ part, int nb, T *ks, int *ki,
+                                                      KSeg S) {
     const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[3] = {KP_RZ, KP_RR, KP_BB};
     __shared__ T pv[4];
     reduce_partials<T, 3>(part, nb, c, sl, ki, pv);
-    const int i = krow<L>();
-    const bool in = i < n;
+    int i[R];
+    bool in[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        i[s] = krow_s<L, R>(s);
+        in[s] = i[s] < n;
+    }
     const T *const vz[2] = {z, it == 0 ? z : pold};
-    T sz[2];
-    row_sums<T, L, 2>(A, i, in, vz, sz);
-    const T zi = in ? z[i] : T(0), poi = (in && it > 0) ? pold[i] : T(0);
+    T sz[R][2];
+    row_sums_r<T, L, 2, R>(A, i, in, vz, sz);
+    T zi[R], poi[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        zi[s] = in[s] ? z[i[s]] : T(0);
+        poi[s] = (in[s] && it > 0) ? pold[i[s]] : T(0);
+    }
     kblock_sync();
     if (pv[3] != T(0)) return;
     const T rho = pv[0];
@@ -351,55 +424,72 @@ __global__ void __launch_bounds__(KNT) k_kcg_spmv(int n, int it, int maxit, T rt
         beta = rho_old != T(0) ? rho / rho_old : T(0);
     }
     if (c.lead && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
-    T acc[1] = {0};
-    if (in && klane<L>() == 0) {
-        const T pi = it == 0 ? zi : zi + beta * poi;
-        const T qi = it == 0 ? sz[0] : sz[0] + beta * sz[1];
-        pnew[i] = pi;
-        q[i] = qi;
-        acc[0] = pi * qi;
+    T acc[R][1];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        acc[s][0] = 0;
+        if (in[s] && klane<L>() == 0) {
+            const T pi = it == 0 ? zi[s] : zi[s] + beta * poi[s];
+            const T qi = it == 0 ? sz[s][0] : sz[s][0] + beta * sz[s][1];
+            pnew[i[s]] = pi;
+            q[i[s]] = qi;
+            acc[s][0] = pi * qi;
+        }
     }
     const int so[1] = {KP_PQ};
-    write_partials<T, 1>(acc, part, nb, so);
+    write_partials<T, 1, R>(acc, part, nb, so);
 }
 
 // launch 2: alpha = rho / p.q; x += alpha p; r -= alpha q; z = dinv r;
-// partials r.z, r.r
-template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kcg_update(int n, int it, T *__restrict__ x, T *__restrict__ r,
-                                                    const T *__restrict__ p, const T *__restrict__ q,
-                                                    const T *__restrict__ dinv, T *__restrict__ z,
-                                                    T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
+// partials r.z, r.r.  The workgroup's KNT/L rows go to its threads in
+// order (row j of the workgroup to thread j mod KNT/R, slab j / (KNT/R)).
+template <typename T, int L, int R = 1>
+__global__ void __launch_bounds__(KNT / R) k_kcg_update(int n, int it, T *__restrict__ x, T *__restrict__ r,
+                                                        const T *__restrict__ p, const T *__restrict__ q,
+                                                        const T *__restrict__ dinv, T *__restrict__ z,
+                                                        T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
+    constexpr int NT = KNT / R, ROWS = KNT / L;
     const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[1] = {KP_PQ};
     __shared__ T pv[2];
     reduce_partials<T, 1>(part, nb, c, sl, ki, pv);
-    const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
-    const bool own = (int)threadIdx.x < KNT / L && i < n;
-    T xi = 0, ri = 0, pi = 0, qi = 0, di = 0;
-    if (own) {
-        xi = x[i];
-        ri = r[i];
-        pi = p[i];
-        qi = q[i];
-        di = dinv[i];
+    int i[R];
+    bool own[R];
+    T xi[R], ri[R], pi[R], qi[R], di[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        const int j = s * NT + (int)threadIdx.x;
+        i[s] = blockIdx.x * ROWS + j;
+        own[s] = j < ROWS && i[s] < n;
+        xi[s] = ri[s] = pi[s] = qi[s] = di[s] = 0;
+        if (own[s]) {
+            xi[s] = x[i[s]];
+            ri[s] = r[i[s]];
+            pi[s] = p[i[s]];
+            qi[s] = q[i[s]];
+            di[s] = dinv[i[s]];
+        }
     }
     kblock_sync();
     if (pv[1] != T(0)) return;
     const T rho = ks[KS_RHO + (it & 1)];
     const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
-    T acc[2] = {0, 0};
-    if (own) {
-        x[i] = xi + alpha * pi;
-        ri = ri - alpha * qi;
-        r[i] = ri;
-        T zi = di * ri;
-        z[i] = zi;
-        acc[0] = ri * zi;
-        acc[1] = ri * ri;
+    T acc[R][2];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        acc[s][0] = acc[s][1] = 0;
+        if (own[s]) {
+            x[i[s]] = xi[s] + alpha * pi[s];
+            const T rn = ri[s] - alpha * qi[s];
+            r[i[s]] = rn;
+            T zn = di[s] * rn;
+            z[i[s]] = zn;
+            acc[s][0] = rn * zn;
+            acc[s][1] = rn * rn;
+        }
     }
     const int so[2] = {KP_RZ, KP_RR};
-    write_partials<T, 2>(acc, part, nb, so);
+    write_partials<T, 2, R>(acc, part, nb, so);
 }
 
 // -------------------------------------------------------------- BiCGStab
@@ -408,53 +498,71 @@ __global__ void __launch_bounds__(KNT) k_kcg_update(int n, int it, T *__restrict
 // iteration it-1, x += alpha dinv p_old + omega dinv s, r = s - omega t,
 // rho = rh.s - omega rh.t, beta = (rho / rho_old)(alpha / omega),
 // p = r + beta (p_old - omega v_old), v = (A D^-1) p.  Partials rh.v, r.r.
-template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol, T atol,
-                                                KRows<T> A, const T *__restrict__ dinv,
-                                                T *__restrict__ x, T *__restrict__ r, const T *__restrict__ rh,
-                                                const T *__restrict__ s, const T *__restrict__ t,
-                                                const T *__restrict__ pold, const T *__restrict__ vold,
-                                                T *__restrict__ pnew, T *__restrict__ vnew, T *__restrict__ part,
-                                                int nb, T *ks, int *ki, KSeg S) {
+template <typename T, int L, int R = 1>
+__global__ void __launch_bounds__(KNT / R) k_kbcg_a(int n, int it, int maxit, T rtol, T atol,
+                                                    KRows<T> A, const T *__restrict__ dinv,
+                                                    T *__restrict__ x, T *__restrict__ r, const T *__restrict__ rh,
+                                                    const T *__restrict__ s, const T *__restrict__ t,
+                                                    const T *__restrict__ pold, const T *__restrict__ vold,
+                                                    T *__restrict__ pnew, T *__restrict__ vnew, T *__restrict__ part,
+                                                    int nb, T *ks, int *ki, KSeg S) {
     const KCtx c = kctx<T>(S, nb, ks, ki);
-    const int i = krow<L>();
-    const bool in = i < n;
-    const bool own = in && klane<L>() == 0;
-    T acc[2] = {0, 0};
+    int i[R];
+    bool in[R], own[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        i[q] = krow_s<L, R>(q);
+        in[q] = i[q] < n;
+        own[q] = in[q] && klane<L>() == 0;
+    }
+    T acc[R][2];
+#pragma unroll
+    for (int q = 0; q < R; q++) acc[q][0] = acc[q][1] = 0;
     if (it == 0) {
         const int sl[3] = {KP_RZ, KP_RR, KP_BB};
         __shared__ T pv[4];
         reduce_partials<T, 3>(part, nb, c, sl, ki, pv);
         const T *const vr[1] = {r};
-        T sr[1];
-        row_sums<T, L, 1>(A, i, in, vr, sr);
-        const T ri = in ? r[i] : T(0), rhi = in ? rh[i] : T(0);
+        T sr[R][1];
+        row_sums_r<T, L, 1, R>(A, i, in, vr, sr);
+        T ri[R], rhi[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            ri[q] = in[q] ? r[i[q]] : T(0);
+            rhi[q] = in[q] ? rh[i[q]] : T(0);
+        }
         kblock_sync();
         if (pv[3] != T(0)) return;
         if (krylov_stop<T>(c.lead, true, 0, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
         if (c.lead && threadIdx.x == 0) ks[KS_RHO] = pv[0];
-        if (own) {
-            pnew[i] = ri;
-            vnew[i] = sr[0];
-            acc[0] = rhi * sr[0];
-            acc[1] = ri * ri;
-        }
+#pragma unroll
+        for (int q = 0; q < R; q++)
+            if (own[q]) {
+                pnew[i[q]] = ri[q];
+                vnew[i[q]] = sr[q][0];
+                acc[q][0] = rhi[q] * sr[q][0];
+                acc[q][1] = ri[q] * ri[q];
+            }
     } else {
         const int sl[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
         __shared__ T pv[5];
         reduce_partials<T, 4>(part, nb, c, sl, ki, pv, KI_BRK + ((it + 1) & 1));
         const T *const vv[4] = {s, t, pold, vold};
-        T sv[4];
-        row_sums<T, L, 4>(A, i, in, vv, sv);
-        T si = 0, ti = 0, poi = 0, voi = 0, xi = 0, di = 0, rhi = 0;
-        if (own) {
-            si = s[i];
-            ti = t[i];
-            poi = pold[i];
-            voi = vold[i];
-            xi = x[i];
-            di = dinv[i];
-            rhi = rh[i];
+        T sv[R][4];
+        row_sums_r<T, L, 4, R>(A, i, in, vv, sv);
+        T si[R], ti[R], poi[R], voi[R], xi[R], di[R], rhi[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            si[q] = ti[q] = poi[q] = voi[q] = xi[q] = di[q] = rhi[q] = 0;
+            if (own[q]) {
+                si[q] = s[i[q]];
+                ti[q] = t[i[q]];
+                poi[q] = pold[i[q]];
+                voi[q] = vold[i[q]];
+                xi[q] = x[i[q]];
+                di[q] = dinv[i[q]];
+                rhi[q] = rh[i[q]];
+            }
         }
         kblock_sync();
         if (pv[4] != T(0)) return;
@@ -464,11 +572,15 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
         const T rho = pv[2] - omega * pv[3];
         const bool breakdown = rho == T(0) || rho_old == T(0) || omega == T(0);
         const T beta = breakdown ? T(0) : (rho / rho_old) * (alpha / omega);
-        T ri = 0;
-        if (own) {
-            x[i] = xi + alpha * (di * poi) + omega * (di * si);
-            ri = si - omega * ti;
-            r[i] = ri;
+        T ri[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            ri[q] = 0;
+            if (own[q]) {
+                x[i[q]] = xi[q] + alpha * (di[q] * poi[q]) + omega * (di[q] * si[q]);
+                ri[q] = si[q] - omega * ti[q];
+                r[i[q]] = ri[q];
+            }
         }
         if (breakdown) {
             // x holds this iteration's update; the host checks the true residual
@@ -480,56 +592,73 @@ __global__ void __launch_bounds__(KNT) k_kbcg_a(int n, int it, int maxit, T rtol
             return;
         }
         if (c.lead && threadIdx.x == 0) ks[KS_RHO + (it & 1)] = rho;
-        if (own) {
-            const T vi = (sv[0] - omega * sv[1]) + beta * (sv[2] - omega * sv[3]);
-            pnew[i] = ri + beta * (poi - omega * voi);
-            vnew[i] = vi;
-            acc[0] = rhi * vi;
-            acc[1] = ri * ri;
-        }
+#pragma unroll
+        for (int q = 0; q < R; q++)
+            if (own[q]) {
+                const T vi = (sv[q][0] - omega * sv[q][1]) + beta * (sv[q][2] - omega * sv[q][3]);
+                pnew[i[q]] = ri[q] + beta * (poi[q] - omega * voi[q]);
+                vnew[i[q]] = vi;
+                acc[q][0] = rhi[q] * vi;
+                acc[q][1] = ri[q] * ri[q];
+            }
     }
     // iteration 0 reads the r.r partials this launch would overwrite (with
     // the same values): it writes its r.r to an unused slot instead
     const int so[2] = {KP_RV, it == 0 ? KP_PQ : KP_RR};
-    write_partials<T, 2>(acc, part, nb, so);
+    write_partials<T, 2, R>(acc, part, nb, so);
 }
 
 // launch B: stopping test on r.r; alpha = rho / rh.v; s = r - alpha v;
 // t = (A D^-1) s; partials t.s, t.t, rh.s, rh.t
-template <typename T, int L>
-__global__ void __launch_bounds__(KNT) k_kbcg_b(int n, int it, int maxit, KRows<T> A,
-                                                const T *__restrict__ r, const T *__restrict__ rh,
-                                                const T *__restrict__ v, T *__restrict__ s, T *__restrict__ t,
-                                                T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
+template <typename T, int L, int R = 1>
+__global__ void __launch_bounds__(KNT / R) k_kbcg_b(int n, int it, int maxit, KRows<T> A,
+                                                    const T *__restrict__ r, const T *__restrict__ rh,
+                                                    const T *__restrict__ v, T *__restrict__ s, T *__restrict__ t,
+                                                    T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
     const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[2] = {KP_RV, KP_RR};
     __shared__ T pv[3];
     reduce_partials<T, 2>(part, nb, c, sl, ki, pv, KI_BRK + (it & 1));
-    const int i = krow<L>();
-    const bool in = i < n;
+    int i[R];
+    bool in[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        i[q] = krow_s<L, R>(q);
+        in[q] = i[q] < n;
+    }
     const T *const vv[2] = {r, v};
-    T sv[2];
-    row_sums<T, L, 2>(A, i, in, vv, sv);
-    const T ri = in ? r[i] : T(0), vi = in ? v[i] : T(0), rhi = in ? rh[i] : T(0);
+    T sv[R][2];
+    row_sums_r<T, L, 2, R>(A, i, in, vv, sv);
+    T ri[R], vi[R], rhi[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        ri[q] = in[q] ? r[i[q]] : T(0);
+        vi[q] = in[q] ? v[i[q]] : T(0);
+        rhi[q] = in[q] ? rh[i[q]] : T(0);
+    }
     kblock_sync();
     if (pv[2] != T(0)) return;
     if (it > 0 && krylov_stop<T>(c.lead, false, it, maxit, pv[1], T(0), T(0), T(0), false, ks, ki)) return;
     const T rho = ks[KS_RHO + (it & 1)];
     const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
     if (c.lead && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;
-    T acc[4] = {0, 0, 0, 0};
-    if (in && klane<L>() == 0) {
-        const T si = ri - alpha * vi;
-        const T ti = sv[0] - alpha * sv[1];
-        s[i] = si;
-        t[i] = ti;
-        acc[0] = ti * si;
-        acc[1] = ti * ti;
-        acc[2] = rhi * si;
-        acc[3] = rhi * ti;
+    T acc[R][4];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0;
+        if (in[q] && klane<L>() == 0) {
+            const T si = ri[q] - alpha * vi[q];
+            const T ti = sv[q][0] - alpha * sv[q][1];
+            s[i[q]] = si;
+            t[i[q]] = ti;
+            acc[q][0] = ti * si;
+            acc[q][1] = ti * ti;
+            acc[q][2] = rhi[q] * si;
+            acc[q][3] = rhi[q] * ti;
+        }
     }
     const int so[4] = {KP_TS, KP_TT, KP_RS, KP_RT};
-    write_partials<T, 4>(acc, part, nb, so);
+    write_partials<T, 4, R>(acc, part, nb, so);
 }
 
 // ---------------------------------------------------------------- driver
