@@ -57,7 +57,7 @@ WORKLOADS = {
     # [0.1, 2]; 1,225,216 cells), beta=0.5, dt=5e-4, semi-implicit Euler,
     # all stepped together (one launch per stage)
     "sweep64_ve_cavity": dict(kind="sweep", count=64, seed=0, dt=5e-4, steps=100,
-                              roof=(12, "k_kcg_spmv<float, 1> (velocity PCG SpMV, mixed form)", "mixed")),
+                              roof=(12, "k_kcg_spmv<float> (velocity PCG SpMV, mixed form)", "mixed")),
 }
 DEFAULT = "sweep64_ve_cavity"
 # launch A of the fused FP64 BiCGStab on the semi-implicit velocity block

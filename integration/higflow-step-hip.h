@@ -3,7 +3,8 @@
  *
  * A driver attaches a set-up higflow_solver once; from then on
  * higflow_solver_step / higflow_solver_step_viscoelastic run on the GPU
- * through the one-line branch each gets.
+ * through a one-line branch at the top of each:
+ *     if (higflow_hip_step(ns) == 0) return;
  */
 #ifndef HIGFLOW_STEP_HIP_H
 #define HIGFLOW_STEP_HIP_H

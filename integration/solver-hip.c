@@ -2,7 +2,7 @@
  * SOLVER_HIP: a HiG-Tree linear-solver engine backed by libhigflow_hip.
  *
  * Drop this file into higtree/src/ next to solver-petsc.c and register it
- * (the three edits are listed in INTEGRATION.md section 1):
+ * with three edits:
  *   - "hip" in engine_names and SOLVER_HIP in SolverEngine
  *     (higtree/src/solver.h:14-39, same position in both lists);
  *   - case SOLVER_HIP: s = _slv_create_hip(first_gid, size); in slv_create
