@@ -45,6 +45,13 @@ class FusedKrylov {
     int ntempl = 0;
     long templ_rows = 0;
     long op_bytes_ = 0;  // the operator as one SpMV launch streams it (one matrix)
+    // diagonal form of the template rows (see KD_MAX), used with one lane
+    // per row
+    KBuf<int> dblk, dmask;
+    KBuf<T> dwv, dwh;
+    bool diag_ = false;
+    long op_bytes_diag_ = 0;
+    long op_bytes() const { return (diag_ && lanes == 1) ? op_bytes_diag_ : op_bytes_; }
     KRows<T> Av, Ah;  // A and A D^-1 as the kernels read them
 
     FusedKrylov() = default;
@@ -248,6 +255,84 @@ class FusedKrylov {
         Av.toff = Ah.toff = toff.p;
         Av.tw = twv.p;
         Ah.tw = twh.p;
+        build_diagonals(code, hcnt, hoff, hwv, hwh, gen_nnz, gen_rows, st);
+    }
+
+    // The diagonal form (KRows::dblk): each segment's diagonals are the
+    // offsets of its longest template; every template must use them in
+    // order in every segment it occurs in, or the matrix keeps the plain
+    // template form.  HFH_DIAG=0 turns it off.
+    void build_diagonals(const std::vector<int> &code, const std::vector<int> &hcnt, const std::vector<int> &hoff,
+                         const std::vector<T> &hwv, const std::vector<T> &hwh, long gen_nnz, long gen_rows,
+                         hipStream_t st) {
+        diag_ = false;
+        if (const char *e = std::getenv("HFH_DIAG"))
+            if (std::strcmp(e, "0") == 0) return;
+        std::vector<int> rseg(n);
+        for (int g = 0; g < nseg; g++)
+            for (int i = seg_off[g]; i < seg_off[g + 1]; i++) rseg[i] = g;
+        for (int g = 1; g < nseg; g++)
+            if (seg_off[g] % KNT) return;  // blocks must not straddle segments
+        std::vector<int> best(nseg, -1);
+        for (int i = 0; i < n; i++) {
+            const int t = code[i], g = rseg[i];
+            if (t >= 0 && (best[g] < 0 || hcnt[t] > hcnt[best[g]])) best[g] = t;
+        }
+        std::vector<std::vector<int>> dg(nseg);
+        for (int g = 0; g < nseg; g++) {
+            if (best[g] < 0) continue;
+            if (hcnt[best[g]] > KD_MAX) return;
+            dg[g].assign(hoff.begin() + (size_t)best[g] * KT_MAX, hoff.begin() + (size_t)best[g] * KT_MAX + hcnt[best[g]]);
+        }
+        // slot of each template entry, checked against every segment the
+        // template occurs in
+        std::vector<int> slot((size_t)ntempl * KT_MAX, -1), seen(ntempl, -1);
+        for (int i = 0; i < n; i++) {
+            const int t = code[i], g = rseg[i];
+            if (t < 0 || seen[t] == g) continue;
+            int pos = 0;
+            for (int e = 0; e < hcnt[t]; e++) {
+                const int off = hoff[(size_t)t * KT_MAX + e];
+                while (pos < (int)dg[g].size() && dg[g][pos] != off) pos++;
+                if (pos == (int)dg[g].size()) return;  // not an in-order subsequence
+                if (seen[t] >= 0 && slot[(size_t)t * KT_MAX + e] != pos) return;
+                slot[(size_t)t * KT_MAX + e] = pos++;
+            }
+            seen[t] = g;
+        }
+        const int nblk = (n + KNT - 1) / KNT;
+        std::vector<int> hb((size_t)nblk * (KD_MAX + 1), 0), hm(ntempl, 0);
+        for (int b = 0; b < nblk; b++) {
+            const std::vector<int> &d = dg[rseg[(size_t)b * KNT]];
+            for (size_t e = 0; e < d.size(); e++) hb[(size_t)b * (KD_MAX + 1) + e] = d[e];
+            hb[(size_t)b * (KD_MAX + 1) + KD_MAX] = (int)d.size();
+        }
+        std::vector<T> wv((size_t)ntempl * KD_MAX, T(0)), wh((size_t)ntempl * KD_MAX, T(0));
+        for (int t = 0; t < ntempl; t++)
+            for (int e = 0; e < hcnt[t]; e++) {
+                const int q = slot[(size_t)t * KT_MAX + e];
+                if (q < 0) continue;  // template unused (cannot happen: every template has rows)
+                hm[t] |= 1 << q;
+                wv[(size_t)t * KD_MAX + q] = hwv[(size_t)t * KT_MAX + e];
+                wh[(size_t)t * KD_MAX + q] = hwh[(size_t)t * KT_MAX + e];
+            }
+        up_int(dblk, hb, st);
+        up_int(dmask, hm, st);
+        up(dwv, wv, st);
+        up(dwh, wh, st);
+        if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("diagonal upload failed");
+        Av.dblk = Ah.dblk = dblk.p;
+        Av.dmask = Ah.dmask = reinterpret_cast<const unsigned *>(dmask.p);
+        Av.dw = dwv.p;
+        Ah.dw = dwh.p;
+        Av.nrows = Ah.nrows = n;
+        diag_ = true;
+        op_bytes_diag_ = (long)n * 4 + gen_nnz * (4 + (long)sizeof(T)) + gen_rows * 8 +
+                         (long)ntempl * (4 + KD_MAX * (long)sizeof(T)) + (long)nblk * (KD_MAX + 1) * 4;
+        if (const char *ev = std::getenv("HFH_VERBOSE"))
+            if (std::strcmp(ev, "1") == 0)
+                std::fprintf(stderr, "hfh: %zu-byte system of %d rows: diagonal form, %zu diagonals in segment 0\n",
+                             sizeof(T), n, dg[0].size());
     }
 
     // r = b - A x and the flags; x is the initial guess.  `cg` asks for
@@ -276,7 +361,7 @@ class FusedKrylov {
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
         float ms = 0;
         with_shape([&](auto Lc, auto Rc) { ms = time_b<Lc.value, Rc.value>(reps, st, S); });
-        *bytes = op_bytes_ + (long)n * 5 * sizeof(T) + (long)nb * 6 * sizeof(T);
+        *bytes = op_bytes() + (long)n * 5 * sizeof(T) + (long)nb * 6 * sizeof(T);
         return ms;
     }
 
@@ -307,7 +392,7 @@ class FusedKrylov {
             throw std::runtime_error("probe set-up failed");
         float ms = 0;
         with_shape([&](auto Lc, auto Rc) { ms = time_a<Lc.value, Rc.value>(reps, st, S); });
-        *bytes = op_bytes_ + (long)n * 11 * sizeof(T) + (long)nb * 6 * sizeof(T);
+        *bytes = op_bytes() + (long)n * 11 * sizeof(T) + (long)nb * 6 * sizeof(T);
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         return ms;
     }
@@ -334,7 +419,7 @@ class FusedKrylov {
             throw std::runtime_error("probe set-up failed");
         float ms = 0;
         with_shape([&](auto Lc, auto Rc) { ms = time_cg<Lc.value, Rc.value>(reps, st); });
-        *bytes = op_bytes_ + (long)n * 4 * sizeof(T) + (long)nb * 4 * sizeof(T);
+        *bytes = op_bytes() + (long)n * 4 * sizeof(T) + (long)nb * 4 * sizeof(T);
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
         return ms;

@@ -236,12 +236,30 @@ __device__ inline int klane() {
 // or code == nullptr, reads the CSR row.  Template rows then cost 4 bytes
 // of operator instead of 4 + 12 nnz (FP64), and sum in the same order.
 constexpr int KT_MAX = 16;
+// Diagonal form of the template rows (uniform-mesh systems): when every
+// template's column offsets are an in-order subsequence of one list of at
+// most KD_MAX offsets per 1024-row block (the block's "diagonals"), a
+// template keeps only its weights in the diagonal slots and a mask of the
+// slots it uses.  The kernels then know where to gather before the row's
+// template id arrives: the offsets come from the block (one uniform load
+// per workgroup), so the vector loads and the id load are in flight
+// together instead of one after the other.  The entries are summed in the
+// same order as the template (and the CSR row).  Used with one lane per
+// row, whose workgroups hold exactly one 1024-row block, by the launches
+// that gather one or two vectors.
+constexpr int KD_MAX = 7;
 template <typename T>
 struct KRows {
     const int *rp = nullptr, *ci = nullptr;
     const T *val = nullptr;
     const int *code = nullptr, *tcnt = nullptr, *toff = nullptr;
     const T *tw = nullptr;
+    // diagonal form: per block KD_MAX offsets and their count (dblk[b *
+    // (KD_MAX + 1) + KD_MAX]); per template a slot mask and KD_MAX weights
+    const int *dblk = nullptr;
+    const unsigned *dmask = nullptr;
+    const T *dw = nullptr;
+    int nrows = 0;
 };
 
 // sums over row i[s]'s entries of val[k] * vec[m][col[k]] for the R rows
@@ -259,14 +277,61 @@ __device__ inline void row_sums_r(const KRows<T> &A, const int (&i)[R], const bo
     int rb[R], re[R], base[R], t[R];
     const int *cp[R];
     const T *vp[R];
+    bool diag = false;
+    // (M <= 2: four gathered vectors in seven slots would cost the FP64
+    // BiCGStab launch A half its occupancy)
+    if constexpr (L == 1 && M <= 2) {
+        if (A.dw) {
+            // diagonal form: gathers first (offsets of this workgroup's
+            // block), then the template ids, then weights and masks
+            diag = true;
+            const int *db = A.dblk + (size_t)blockIdx.x * (KD_MAX + 1);
+            const int nd = db[KD_MAX];
+            T xv[R][KD_MAX][M];
 #pragma unroll
-    for (int r = 0; r < R; r++) t[r] = (in[r] && A.code) ? A.code[i[r]] : -1;
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int e = 0; e < KD_MAX; e++) {
+                    int j = i[r] + db[e];
+                    j = j < 0 ? 0 : (j >= A.nrows ? A.nrows - 1 : j);
+#pragma unroll
+                    for (int m = 0; m < M; m++) xv[r][e][m] = (in[r] && e < nd) ? vec[m][j] : T(0);
+                }
+#pragma unroll
+            for (int r = 0; r < R; r++) t[r] = in[r] ? A.code[i[r]] : -1;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+#pragma unroll
+                for (int m = 0; m < M; m++) out[r][m] = 0;
+                if (t[r] >= 0) {
+                    const unsigned msk = A.dmask[t[r]];
+                    const T *w = A.dw + (size_t)t[r] * KD_MAX;
+#pragma unroll
+                    for (int e = 0; e < KD_MAX; e++) {
+                        const T a = w[e];
+#pragma unroll
+                        for (int m = 0; m < M; m++)
+                            if ((msk >> e) & 1u) out[r][m] += a * xv[r][e][m];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (!diag) t[r] = (in[r] && A.code) ? A.code[i[r]] : -1;
     int len = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
+        if (!diag) {
 #pragma unroll
-        for (int m = 0; m < M; m++) out[r][m] = 0;
-        if (t[r] >= 0) {
+            for (int m = 0; m < M; m++) out[r][m] = 0;
+        }
+        if (diag && t[r] >= 0) {  // summed above
+            rb[r] = re[r] = base[r] = 0;
+            cp[r] = A.ci;
+            vp[r] = A.val;
+        } else if (t[r] >= 0) {
             rb[r] = t[r] * KT_MAX;
             re[r] = rb[r] + A.tcnt[t[r]];
             cp[r] = A.toff;

@@ -6,7 +6,8 @@ for every R.  Whole time steps with R forced through HFH_KRYLOV_ROWS (and
 the lanes per row through HFH_KRYLOV_LANES) must agree bit for bit, for the
 Jacobi-PCG velocity systems, the BiCGStab velocity systems of an AMR mesh,
 the AMG-BiCGStab and Jacobi-PCG pressure solves, the FP32 corrections of
-the mixed form and the segmented systems of a batch.
+the mixed form and the segmented systems of a batch.  The diagonal form of
+the template rows is checked the same way against the plain template rows.
 """
 import numpy as np
 import pytest
@@ -66,3 +67,32 @@ def test_rows_per_lane_group_leave_the_iterates_unchanged(which, precision, lane
         for k, (a, b) in enumerate(zip(runs[1], runs[rows])):
             assert np.all(np.isfinite(a))
             assert np.array_equal(a, b), (which, precision, lanes, rows, k, np.abs(a - b).max())
+
+
+def _case3d():
+    return cases.cavity(12, dim=3, Re=10.0, dt=1e-3, tempdiscrtype=cases.SEMI_IMPLICIT_EULER, **TOL)
+
+
+@pytest.mark.parametrize("which,precision", [("ve_cavity", 0), ("cavity_pcg", 0), ("ve_cavity", 1), ("batch", 0),
+                                             ("batch", 1), ("cavity3d", 0)])
+def test_diagonal_form_leaves_the_iterates_unchanged(which, precision, monkeypatch):
+    """Template rows in the diagonal form (gather offsets per 1024-row block,
+    one lane per row) sum the same entries in the same order as the plain
+    template rows (HFH_DIAG=0)."""
+    monkeypatch.setenv("HFH_KRYLOV_LANES", "1")
+    if which == "cavity_pcg":
+        monkeypatch.setenv("HFH_AMG", "0")
+    runs = []
+    for diag in ("1", "0"):
+        monkeypatch.setenv("HFH_DIAG", diag)
+        if which == "cavity3d":
+            sim = native.Flow(_case3d())
+            for _ in range(4):
+                sim.step()
+            runs.append([sim.get(0, d) for d in range(3)] + [sim.get(1, 0)])
+        else:
+            runs.append(_run(which, precision))
+    for other in runs[1:]:
+        for k, (a, b) in enumerate(zip(runs[0], other)):
+            assert np.all(np.isfinite(a))
+            assert np.array_equal(a, b), (which, precision, k, np.abs(a - b).max())
