@@ -32,6 +32,8 @@ class FusedKrylov {
     // (requested before set(); used when the hierarchy builds)
     bool want_amg = false;
     AmgPrecond<T> amg;
+    // symmetric systems with a hierarchy: AMG-PCG (HFH_AMG_PCG=0: AMG-BiCGStab)
+    bool amg_pcg = true;
     KBuf<T> ph, sh;
     // segments (block-diagonal systems): row offsets, multiples of 1024
     int nseg = 1;
@@ -99,6 +101,10 @@ class FusedKrylov {
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         k_fill<int><<<nblocks((long)KI_N * nseg), BLOCK, 0, st>>>(KI_N * nseg, ki.p, 0);
         amg.ready = false;
+        {
+            const char *e = std::getenv("HFH_AMG_PCG");
+            amg_pcg = !(e && std::strcmp(e, "0") == 0);
+        }
         if (want_amg && amg.build(n, hrp, hci, hv, st)) {
             ph.alloc(n);
             sh.alloc(n);
@@ -336,8 +342,10 @@ class FusedKrylov {
     }
 
     // r = b - A x and the flags; x is the initial guess.  `cg` asks for
-    // Jacobi-PCG (symmetric positive definite systems); with a V-cycle the
-    // solve is AMG-BiCGStab in either case.
+    // PCG (symmetric positive definite systems): Jacobi-PCG, or with a
+    // V-cycle AMG-PCG (one symmetric V(1,1) cycle per iteration); without
+    // `cg` the solve is Jacobi- or AMG-BiCGStab.  AMG-PCG starts as
+    // BiCGStab does (r, r.r, b.b); its z comes from the first V-cycle.
     template <typename TB>
     void init(bool cg, const TB *b, T *x, hipStream_t st, bool xzero = false) {
         cg = cg && !amg.ready;
@@ -345,7 +353,7 @@ class FusedKrylov {
     }
     // iterations [it0, it0 + count)
     void iterate(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
-        cg = cg && !amg.ready;
+        cg = cg && (!amg.ready || amg_pcg);
         with_shape([&](auto Lc, auto Rc) {
             launch_iter<Lc.value, Rc.value>(cg, it0, count, maxit, rtol, atol, x, st);
         });
@@ -485,6 +493,20 @@ class FusedKrylov {
     // the V-cycle path keeps R = 1 (its launches are bound by the cycle)
     template <int L, int R>
     void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
+        if (amg.ready && cg) {
+            // AMG-PCG: z = V(r), rho = r.z, then the two PCG launches
+            const int *kc = nseg == 1 ? ki.p : nullptr;
+            for (int it = it0; it < it0 + count; it++) {
+                T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
+                amg.apply(r.p, z.p, kc, st);
+                k_krz<T, L><<<nb, KNT, 0, st>>>(n, r.p, z.p, part.p, nb, ki.p, S);
+                k_kcg_spmv<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, maxit, rtol, atol, Av, z.p, pold, pnew, q.p,
+                                                            part.p, nb, ks.p, ki.p, S);
+                k_kcg_update<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, x, r.p, pnew, q.p, dinv.p, z.p, part.p, nb,
+                                                              ks.p, ki.p, S, 0);
+            }
+            return;
+        }
         if (amg.ready) {
             // one segment: the cycle stops with the solve; several: it runs
             // until the host sees every segment done

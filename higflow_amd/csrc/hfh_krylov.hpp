@@ -508,11 +508,14 @@ __global__ void __launch_bounds__(KNT / R) k_kcg_spmv(int n, int it, int maxit, 
 // launch 2: alpha = rho / p.q; x += alpha p; r -= alpha q; z = dinv r;
 // partials r.z, r.r.  The workgroup's KNT/L rows go to its threads in
 // order (row j of the workgroup to thread j mod KNT/R, slab j / (KNT/R)).
+// jacobi = 0 (AMG-PCG): z is left to the V-cycle and the r.z partial to
+// k_krz (the slot is written with zeros here).
 template <typename T, int L, int R = 1>
 __global__ void __launch_bounds__(KNT / R) k_kcg_update(int n, int it, T *__restrict__ x, T *__restrict__ r,
                                                         const T *__restrict__ p, const T *__restrict__ q,
                                                         const T *__restrict__ dinv, T *__restrict__ z,
-                                                        T *__restrict__ part, int nb, T *ks, int *ki, KSeg S) {
+                                                        T *__restrict__ part, int nb, T *ks, int *ki, KSeg S,
+                                                        int jacobi = 1) {
     constexpr int NT = KNT / R, ROWS = KNT / L;
     const KCtx c = kctx<T>(S, nb, ks, ki);
     const int sl[1] = {KP_PQ};
@@ -532,7 +535,7 @@ __global__ void __launch_bounds__(KNT / R) k_kcg_update(int n, int it, T *__rest
             ri[s] = r[i[s]];
             pi[s] = p[i[s]];
             qi[s] = q[i[s]];
-            di[s] = dinv[i[s]];
+            di[s] = jacobi ? dinv[i[s]] : T(0);
         }
     }
     kblock_sync();
@@ -547,14 +550,31 @@ __global__ void __launch_bounds__(KNT / R) k_kcg_update(int n, int it, T *__rest
             x[i[s]] = xi[s] + alpha * pi[s];
             const T rn = ri[s] - alpha * qi[s];
             r[i[s]] = rn;
-            T zn = di[s] * rn;
-            z[i[s]] = zn;
-            acc[s][0] = rn * zn;
+            if (jacobi) {
+                T zn = di[s] * rn;
+                z[i[s]] = zn;
+                acc[s][0] = rn * zn;
+            }
             acc[s][1] = rn * rn;
         }
     }
     const int so[2] = {KP_RZ, KP_RR};
     write_partials<T, 2, R>(acc, part, nb, so);
+}
+
+// AMG-PCG: partial r.z of the preconditioned residual z = V(r), the rho of
+// the next k_kcg_spmv (rows as in k_kcg_update); returns at once when the
+// segment's solve has stopped.
+template <typename T, int L>
+__global__ void __launch_bounds__(KNT) k_krz(int n, const T *__restrict__ r, const T *__restrict__ z,
+                                             T *__restrict__ part, int nb, const int *ki, KSeg S) {
+    if (S.wseg) ki += (size_t)S.wseg[blockIdx.x] * KI_N;
+    if (ki[KI_DONE]) return;
+    const int i = blockIdx.x * (KNT / L) + (int)threadIdx.x;
+    const bool own = (int)threadIdx.x < KNT / L && i < n;
+    T acc[1] = {own ? r[i] * z[i] : T(0)};
+    const int so[1] = {KP_RZ};
+    write_partials<T, 1>(acc, part, nb, so);
 }
 
 // -------------------------------------------------------------- BiCGStab
