@@ -304,6 +304,7 @@ def main():
         "poisson_iters_fp64": iters,
         "err_mixed_vs_fp64": errs,
         "stage_ms_fp64": {k2: round(v2, 4) for k2, v2 in st64.items()},
+        "stage_ms_mixed": {k2: round(v2, 4) for k2, v2 in stmx.items()},
         "solves_last_step": ({"fp64": [sim64.solve_stats(w) for w in range(D + 1)],
                               "mixed": [simmx.solve_stats(w) for w in range(D + 1)]} if not batch else
                              {"fp64_max_iters": [max(sim64.solve_stats(m, w)["iters"] for m in range(sim64.n))

@@ -292,11 +292,13 @@ __device__ inline T arow(int rb, int re, const int *__restrict__ ci, const T *__
     return s;
 }
 
-// r = b - A (omega D^-1 b)
-template <typename T>
+// r = b - A (omega D^-1 b).  The fine level's b may be of another type TB
+// (an FP64 Krylov solve preconditioned by the FP32 cycle); the cycle
+// computes in T.
+template <typename T, typename TB = T>
 __global__ void __launch_bounds__(ANT) k_amg_resid(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                    const T *__restrict__ v, const T *__restrict__ dinv, T omega,
-                                                   const T *__restrict__ b, T *__restrict__ r, const int *ki) {
+                                                   const TB *__restrict__ b, T *__restrict__ r, const int *ki) {
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < n;
@@ -304,11 +306,11 @@ __global__ void __launch_bounds__(ANT) k_amg_resid(int n, const int *__restrict_
     T s = 0;
     for (int k = rb + (int)(threadIdx.x % AL); k < re; k += AL) {
         const int j = ci[k];
-        s += v[k] * (omega * (dinv[j] * b[j]));
+        s += v[k] * (omega * (dinv[j] * (T)b[j]));
     }
 #pragma unroll
     for (int o = AL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (in && threadIdx.x % AL == 0) r[i] = b[i] - s;
+    if (in && threadIdx.x % AL == 0) r[i] = (T)b[i] - s;
 }
 
 // b_c = R r
@@ -366,42 +368,42 @@ __global__ void __launch_bounds__(ANT) k_amg_dense(int nc, const TC *__restrict_
 }
 
 // x = omega D^-1 b + P x_c
-template <typename T>
+template <typename T, typename TB = T>
 __global__ void __launch_bounds__(ANT) k_amg_prolong(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                      const T *__restrict__ v, const T *__restrict__ dinv, T omega,
-                                                     const T *__restrict__ b, const T *__restrict__ xc,
+                                                     const TB *__restrict__ b, const T *__restrict__ xc,
                                                      T *__restrict__ x, const int *ki) {
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < n;
     const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, xc);
-    if (in && threadIdx.x % AL == 0) x[i] = omega * (dinv[i] * b[i]) + s;
+    if (in && threadIdx.x % AL == 0) x[i] = omega * (dinv[i] * (T)b[i]) + s;
 }
 
 // y = x + omega D^-1 (b - A x)
-template <typename T>
+template <typename T, typename TB = T, typename TY = T>
 __global__ void __launch_bounds__(ANT) k_amg_post(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const T *__restrict__ v, const T *__restrict__ dinv, T omega,
-                                                  const T *__restrict__ b, const T *__restrict__ x,
-                                                  T *__restrict__ y, const int *ki) {
+                                                  const TB *__restrict__ b, const T *__restrict__ x,
+                                                  TY *__restrict__ y, const int *ki) {
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < n;
     const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, x);
-    if (in && threadIdx.x % AL == 0) y[i] = x[i] + omega * (dinv[i] * (b[i] - s));
+    if (in && threadIdx.x % AL == 0) y[i] = (TY)(x[i] + omega * (dinv[i] * ((T)b[i] - s)));
 }
 
 // y = omega D^-1 b + P x_c + omega D^-1 (r - (A P) x_c): prolongation and
 // post-smoothing of one level in one launch.  r = b - A (omega D^-1 b) from
 // the level's residual launch, so for x = omega D^-1 b + P x_c the smoother's
 // b - A x is r - (A P) x_c, with A P stored at set-up.
-template <typename T>
+template <typename T, typename TB = T, typename TY = T>
 __global__ void __launch_bounds__(ANT) k_amg_prolong_post(int n, const int *__restrict__ prp,
                                                           const int *__restrict__ pci, const T *__restrict__ pv,
                                                           const int *__restrict__ qrp, const int *__restrict__ qci,
                                                           const T *__restrict__ qv, const T *__restrict__ dinv,
-                                                          T omega, const T *__restrict__ b, const T *__restrict__ r,
-                                                          const T *__restrict__ xc, T *__restrict__ y,
+                                                          T omega, const TB *__restrict__ b, const T *__restrict__ r,
+                                                          const T *__restrict__ xc, TY *__restrict__ y,
                                                           const int *ki) {
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
@@ -410,7 +412,7 @@ __global__ void __launch_bounds__(ANT) k_amg_prolong_post(int n, const int *__re
     const T s2 = arow<T>(in ? qrp[i] : 0, in ? qrp[i + 1] : 0, qci, qv, xc);
     if (in && threadIdx.x % AL == 0) {
         const T di = dinv[i];
-        y[i] = (omega * (di * b[i]) + s1) + omega * (di * (r[i] - s2));
+        y[i] = (TY)((omega * (di * (T)b[i]) + s1) + omega * (di * (r[i] - s2)));
     }
 }
 
@@ -491,8 +493,13 @@ class AmgPrecond {
         return true;
     }
 
-    // y = V(b) on the fine level; `ki` is the solve's flag array
-    void apply(const T *b, T *y, const int *ki, hipStream_t st) { cycle(0, b, y, ki, st); }
+    // y = V(b) on the fine level; `ki` is the solve's flag array.  b and y
+    // may be of another precision than the cycle (TB, TY): the fine
+    // level's first and last launches convert.
+    template <typename TB, typename TY>
+    void apply(const TB *b, TY *y, const int *ki, hipStream_t st) {
+        cycle<TB, TY>(0, b, y, ki, st);
+    }
 
     long bytes_per_cycle() const {
         long by = 0;
@@ -529,12 +536,13 @@ class AmgPrecond {
     static int grid(int rows) { return std::max(1, (rows * AL + ANT - 1) / ANT); }
 
     // y = V(b) at level l (y, b of that level's size)
-    void cycle(int l, const T *b, T *y, const int *ki, hipStream_t st) {
+    template <typename TB, typename TY>
+    void cycle(int l, const TB *b, TY *y, const int *ki, hipStream_t st) {
         Dev &d = lv_[l];
-        k_amg_resid<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.r.p, ki);
+        k_amg_resid<T, TB><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.r.p, ki);
         if (l + 1 < nlev) {
             k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
-            cycle(l + 1, d.bnext.p, d.xnext.p, ki, st);
+            cycle<T, T>(l + 1, d.bnext.p, d.xnext.p, ki, st);
         } else if (ncoarse <= 256) {
             k_amg_coarse<T><<<1, 1024, ncoarse * sizeof(T), st>>>(ncoarse, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, cinv_.p,
                                                                    d.xnext.p, ki);
@@ -548,13 +556,15 @@ class AmgPrecond {
                 ncoarse, cinv32_.p, d.bnext.p, d.xnext.p, ki);
         }
         if (fuse_pp) {
-            k_amg_prolong_post<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p, d.Qv.p,
-                                                             d.dinv.p, omega, b, d.r.p, d.xnext.p, y, ki);
+            k_amg_prolong_post<T, TB, TY><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p,
+                                                                     d.Qv.p, d.dinv.p, omega, b, d.r.p, d.xnext.p,
+                                                                     y, ki);
             return;
         }
-        k_amg_prolong<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b, d.xnext.p,
-                                                    d.x.p, ki);
-        k_amg_post<T><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.x.p, y, ki);
+        k_amg_prolong<T, TB><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b,
+                                                        d.xnext.p, d.x.p, ki);
+        k_amg_post<T, TB, TY><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.x.p, y,
+                                                         ki);
     }
 };
 

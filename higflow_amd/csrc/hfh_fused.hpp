@@ -34,6 +34,10 @@ class FusedKrylov {
     AmgPrecond<T> amg;
     // symmetric systems with a hierarchy: AMG-PCG (HFH_AMG_PCG=0: AMG-BiCGStab)
     bool amg_pcg = true;
+    // an FP32 hierarchy of the same matrix that an FP64 solve may use as its
+    // preconditioner (the mixed form's FP64 candidate); set by the owner
+    AmgPrecond<float> *amg32 = nullptr;
+    bool use_amg32 = false;
     KBuf<T> ph, sh;
     // segments (block-diagonal systems): row offsets, multiples of 1024
     int nseg = 1;
@@ -111,6 +115,14 @@ class FusedKrylov {
         }
     }
     bool amg_on() const { return amg.ready; }
+
+    // y = V(b): this solver's hierarchy, or the FP32 one when use_amg32
+    void precondition(const T *b, T *y, const int *kc, hipStream_t st) {
+        if (use_amg32 && amg32 && amg32->ready && amg32->nlev > 0)
+            amg32->apply<T, T>(b, y, kc, st);
+        else
+            amg.template apply<T, T>(b, y, kc, st);
+    }
 
     // workgroup -> segment tables for L lanes per row (workgroups of
     // KNT / L rows; segment offsets are multiples of KNT rows)
@@ -498,7 +510,7 @@ class FusedKrylov {
             const int *kc = nseg == 1 ? ki.p : nullptr;
             for (int it = it0; it < it0 + count; it++) {
                 T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
-                amg.apply(r.p, z.p, kc, st);
+                precondition(r.p, z.p, kc, st);
                 k_krz<T, L><<<nb, KNT, 0, st>>>(n, r.p, z.p, part.p, nb, ki.p, S);
                 k_kcg_spmv<T, L, R><<<nb, KNT / R, 0, st>>>(n, it, maxit, rtol, atol, Av, z.p, pold, pnew, q.p,
                                                             part.p, nb, ks.p, ki.p, S);
@@ -514,11 +526,11 @@ class FusedKrylov {
             for (int it = it0; it < it0 + count; it++) {
                 k_abcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, s.p, t.p, ph.p, sh.p, p.p, v.p, part.p, nb, ks.p,
                                                    ki.p, S);
-                amg.apply(p.p, ph.p, kc, st);
+                precondition(p.p, ph.p, kc, st);
                 k_abcg_b<T, L><<<nb, KNT, 0, st>>>(n, it, maxit, rtol, atol, Av, ph.p, rh.p, v.p, part.p,
                                                    nb, ks.p, ki.p, S);
                 k_abcg_c<T, L><<<nb, KNT, 0, st>>>(n, it, r.p, v.p, s.p, part.p, nb, ks.p, ki.p, S);
-                amg.apply(s.p, sh.p, kc, st);
+                precondition(s.p, sh.p, kc, st);
                 k_abcg_d<T, L><<<nb, KNT, 0, st>>>(n, it, Av, sh.p, s.p, rh.p, t.p, part.p, nb, ki.p, S);
             }
             return;

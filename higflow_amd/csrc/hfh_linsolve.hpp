@@ -455,6 +455,7 @@ class DeviceSystem {
     int off_ks_ = 64, off_sc_ = 256;
     size_t hpin_size_ = 0;
     bool force32_ = false;  // HFH_MIXED_FP32=1: always refine with FP32 solves
+    bool mixed_amg32_ = true;  // the mixed form's FP64 solves use the FP32 V-cycle
     int pred32_[3] = {0, 0, 0};  // FP32 iterations of the 1st, 2nd, later corrections
     DevBuf<int> bseg_;              // segment of each BLOCK-row block (nseg > 1)
     DevBuf<double> segsc_;          // per-segment scale s_g and unscale u_g
@@ -542,6 +543,13 @@ class DeviceSystem {
         }
         k64.set(n, rp.p, ci.p, hrp, hci, hv, st, segs);
         k32.set(n, rp.p, ci.p, hrp, hci, hv, st, segs);
+        // the mixed form's FP64 candidate preconditions with the FP32
+        // hierarchy (HFH_MIXED_AMG32=0: with its own FP64 one)
+        k64.amg32 = &k32.amg;
+        {
+            const char *e = std::getenv("HFH_MIXED_AMG32");
+            mixed_amg32_ = !(e && std::strcmp(e, "0") == 0);
+        }
         // HFH_KRYLOV=classic selects the one-kernel-per-operation iterations
         const char *e = std::getenv("HFH_KRYLOV");
         fused = !(e && std::strcmp(e, "classic") == 0);
@@ -707,7 +715,9 @@ class DeviceSystem {
                 S = F;
             }
         } else {
+            k64.use_amg32 = mixed_amg32_;
             S = fused64(spd, b, x, rtol, atol, maxit, st);
+            k64.use_amg32 = false;
             S.fallback = 2;
             for (auto &g : seg_stats) g.fallback = 2;
         }
