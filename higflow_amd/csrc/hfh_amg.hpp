@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -148,6 +149,12 @@ inline int sa_aggregate(const HCsr &A, double theta, std::vector<int> &agg) {
         for (int k = A.rp[i]; k < A.rp[i + 1] && iso; k++) iso = A.ci[k] == i;
         if (iso) agg[i] = -2;
     }
+    // rows with no strong neighbour (a diagonally dominant row, as in the
+    // Helmholtz rows of a high-Re, coarse-mesh simulation) are left to the
+    // smoother too: as singleton aggregates they would keep every such row
+    // on every level and widen its stencil each time
+    for (int i = 0; i < n; i++)
+        if (agg[i] == -1 && S[i].empty()) agg[i] = -2;
     int na = 0;
     for (int i = 0; i < n; i++) {
         if (agg[i] != -1) continue;
@@ -184,11 +191,16 @@ struct AmgHostLevel {
 
 // Builds the hierarchy; returns false (no hierarchy) when the coarsening
 // stalls or the coarsest matrix cannot be inverted.
+// max_cx > 0: give up as soon as the operator complexity (sum over levels of
+// the matrix entries / the fine matrix's) exceeds it.
 inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<double> &cinv, int &nc,
-                     int coarse_max = 400, double theta = 0.08, int max_levels = 10) {
+                     int coarse_max = 400, double theta = 0.08, int max_levels = 10, double max_cx = 0) {
     lv.clear();
     HCsr A = A0;
+    double nz = 0;
     while (A.nrow > coarse_max && (int)lv.size() < max_levels) {
+        nz += (double)A.v.size();
+        if (max_cx > 0 && nz > max_cx * (double)std::max<size_t>(A0.v.size(), 1)) return false;
         AmgHostLevel L;
         L.A = A;
         L.dinv = csr_diag_inv(A);
@@ -423,13 +435,16 @@ class AmgPrecond {
   public:
     bool ready = false;
     int nlev = 0, ncoarse = 0;
+    double complexity = 0;
     T omega = T(0.8);
     // prolongation and post-smoothing fused through A P (HFH_AMG_FUSE=0:
     // two launches per level, as x += P x_c, x += omega D^-1 (b - A x))
     bool fuse_pp = true;
 
+    // max_cx > 0: give up (no hierarchy) when the operator complexity
+    // sum_l nnz(A_l) / nnz(A_0) exceeds it
     bool build(int n, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<double> &v,
-               hipStream_t st) {
+               hipStream_t st, double max_cx = 0) {
         HCsr A;
         A.nrow = A.ncol = n;
         A.rp = rp;
@@ -446,7 +461,24 @@ class AmgPrecond {
         double theta = 0.08;
         if (const char *e = std::getenv("HFH_AMG_THETA")) theta = std::atof(e);
         if (const char *e = std::getenv("HFH_AMG_OMEGA")) omega = (T)std::atof(e);
-        if (!sa_build(A, hl, cinv, ncoarse, 1000, theta)) return false;
+        if (!sa_build(A, hl, cinv, ncoarse, 1000, theta, 10, max_cx)) {
+            if (const char *ev = std::getenv("HFH_VERBOSE"))
+                if (std::strcmp(ev, "1") == 0)
+                    std::fprintf(stderr, "hfh: AMG over %d rows: no hierarchy (coarsening stalled or operator "
+                                 "complexity above %.1f)\n", n, max_cx);
+            return false;
+        }
+        {
+            double nz = 0;
+            for (const AmgHostLevel &L : hl) nz += (double)L.A.v.size();
+            complexity = hl.empty() || hl[0].A.v.empty() ? 0.0 : nz / (double)hl[0].A.v.size();
+            if (const char *ev = std::getenv("HFH_VERBOSE"))
+                if (std::strcmp(ev, "1") == 0)
+                    std::fprintf(stderr, "hfh: AMG over %d rows: %zu levels, coarsest %d, operator complexity %.2f%s\n",
+                                 n, hl.size(), ncoarse, complexity,
+                                 max_cx > 0 && complexity > max_cx ? " (not used)" : "");
+            if (max_cx > 0 && complexity > max_cx) return false;
+        }
         {
             const char *e = std::getenv("HFH_AMG_FUSE");
             fuse_pp = !(e && std::strcmp(e, "0") == 0);

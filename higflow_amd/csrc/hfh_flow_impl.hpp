@@ -244,7 +244,15 @@ void Flow<D>::upload_device() {
                 segs.push_back((int)rp.size() - 1);
             }
             hoff_[D] = (int)rp.size() - 1;
-            hsysB_.set_matrix(rp, ci, v, st_, false, segs);
+            // symmetric velocity systems (uniform meshes) get an AMG-PCG when
+            // their hierarchy stays sparse (operator complexity <= 2; the WLS
+            // rows of AMR meshes give about 6, where the V-cycle costs more
+            // than the Jacobi iterations it saves); HFH_VEL_AMG=0 turns it off
+            bool vamg = true;
+            for (int d = 0; d < D; d++) vamg = vamg && Hsym_[d];
+            if (const char *e = std::getenv("HFH_VEL_AMG"))
+                if (std::strcmp(e, "0") == 0) vamg = false;
+            hsysB_.set_matrix(rp, ci, v, st_, vamg, segs, 2.0);
             hbB_.alloc(hoff_[D]);
             hxB_.alloc(hoff_[D]);
             hxPrev_.alloc(hoff_[D]);

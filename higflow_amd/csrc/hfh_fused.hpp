@@ -31,6 +31,7 @@ class FusedKrylov {
     // BiCGStab right-preconditioned by a smoothed-aggregation V-cycle
     // (requested before set(); used when the hierarchy builds)
     bool want_amg = false;
+    double amg_max_cx = 0;  // > 0: no hierarchy above this operator complexity
     AmgPrecond<T> amg;
     // symmetric systems with a hierarchy: AMG-PCG (HFH_AMG_PCG=0: AMG-BiCGStab)
     bool amg_pcg = true;
@@ -109,7 +110,7 @@ class FusedKrylov {
             const char *e = std::getenv("HFH_AMG_PCG");
             amg_pcg = !(e && std::strcmp(e, "0") == 0);
         }
-        if (want_amg && amg.build(n, hrp, hci, hv, st)) {
+        if (want_amg && amg.build(n, hrp, hci, hv, st, amg_max_cx)) {
             ph.alloc(n);
             sh.alloc(n);
         }

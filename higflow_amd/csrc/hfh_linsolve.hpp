@@ -475,8 +475,10 @@ class DeviceSystem {
     // V-cycle (HFH_AMG=0 turns it off)
     // `segs`: row offsets of independent diagonal blocks (multiples of 1024;
     // see set_blocks), solved together with one stopping test each
+    // `amg_max_cx` > 0: the hierarchy is kept only up to that operator
+    // complexity (sum over levels of the matrix entries / the fine matrix's)
     void set_matrix(const std::vector<int> &hrp, const std::vector<int> &hci, const std::vector<double> &hv,
-                    hipStream_t st, bool amg = false, const std::vector<int> &segs = {}) {
+                    hipStream_t st, bool amg = false, const std::vector<int> &segs = {}, double amg_max_cx = 0) {
         n = (int)hrp.size() - 1;
         nnz = (int)hci.size();
         rp.upload(hrp.data(), hrp.size(), st);
@@ -530,6 +532,7 @@ class DeviceSystem {
         {
             const char *ea = std::getenv("HFH_AMG");
             k64.want_amg = k32.want_amg = amg && !(ea && std::strcmp(ea, "0") == 0);
+            k64.amg_max_cx = k32.amg_max_cx = amg_max_cx;
         }
         t_mixed_ = t_fp64_ = -1;
         n32_ = n64_ = 0;
