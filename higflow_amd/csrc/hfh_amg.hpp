@@ -325,6 +325,44 @@ __global__ void __launch_bounds__(ANT) k_amg_resid(int n, const int *__restrict_
     if (in && threadIdx.x % AL == 0) r[i] = (T)b[i] - s;
 }
 
+// The fine level's residual launch when the matrix has the diagonal form
+// of the Krylov kernels (KRows::dblk): one row per thread in 1024-row
+// workgroups, the gathers of omega D^-1 b at the block's diagonals issued
+// before the row's template id arrives; rows without a template read CSR.
+template <typename T, typename TB = T>
+__global__ void __launch_bounds__(KNT) k_amg_resid_d(int n, KRows<T> A, const T *__restrict__ dinv, T omega,
+                                                     const TB *__restrict__ b, T *__restrict__ r, const int *ki) {
+    if (amg_done(ki)) return;
+    const int i = blockIdx.x * KNT + (int)threadIdx.x;
+    const bool in = i < n;
+    const int *db = A.dblk + (size_t)blockIdx.x * (KD_MAX + 1);
+    const int nd = db[KD_MAX];
+    T xv[KD_MAX];
+#pragma unroll
+    for (int e = 0; e < KD_MAX; e++) {
+        int j = i + db[e];
+        j = j < 0 ? 0 : (j >= n ? n - 1 : j);
+        xv[e] = (in && e < nd) ? omega * (dinv[j] * (T)b[j]) : T(0);
+    }
+    const int t = in ? A.code[i] : -1;
+    T s = 0;
+    if (t >= 0) {
+        const unsigned msk = A.dmask[t];
+        const T *w = A.dw + (size_t)t * KD_MAX;
+#pragma unroll
+        for (int e = 0; e < KD_MAX; e++) {
+            const T a = w[e];
+            if ((msk >> e) & 1u) s += a * xv[e];
+        }
+    } else if (in) {
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int j = A.ci[k];
+            s += A.val[k] * (omega * (dinv[j] * (T)b[j]));
+        }
+    }
+    if (in) r[i] = (T)b[i] - s;
+}
+
 // b_c = R r
 template <typename T>
 __global__ void __launch_bounds__(ANT) k_amg_restrict(int nc, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -436,6 +474,9 @@ class AmgPrecond {
     bool ready = false;
     int nlev = 0, ncoarse = 0;
     double complexity = 0;
+    // the fine matrix as the Krylov kernels read it, when it has the
+    // diagonal form (set by the owner; the fine residual launch then uses it)
+    KRows<T> A0;
     T omega = T(0.8);
     // prolongation and post-smoothing fused through A P (HFH_AMG_FUSE=0:
     // two launches per level, as x += P x_c, x += omega D^-1 (b - A x))
@@ -571,7 +612,11 @@ class AmgPrecond {
     template <typename TB, typename TY>
     void cycle(int l, const TB *b, TY *y, const int *ki, hipStream_t st) {
         Dev &d = lv_[l];
-        k_amg_resid<T, TB><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.r.p, ki);
+        if (l == 0 && A0.dw)
+            k_amg_resid_d<T, TB><<<(d.n + KNT - 1) / KNT, KNT, 0, st>>>(d.n, A0, d.dinv.p, omega, b, d.r.p, ki);
+        else
+            k_amg_resid<T, TB><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.r.p,
+                                                          ki);
         if (l + 1 < nlev) {
             k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
             cycle<T, T>(l + 1, d.bnext.p, d.xnext.p, ki, st);

@@ -110,7 +110,10 @@ class FusedKrylov {
             const char *e = std::getenv("HFH_AMG_PCG");
             amg_pcg = !(e && std::strcmp(e, "0") == 0);
         }
+        amg.A0 = KRows<T>();
         if (want_amg && amg.build(n, hrp, hci, hv, st, amg_max_cx)) {
+            if (diag_ && !(std::getenv("HFH_AMG_DIAG") && std::strcmp(std::getenv("HFH_AMG_DIAG"), "0") == 0))
+                amg.A0 = Av;
             ph.alloc(n);
             sh.alloc(n);
         }

@@ -78,8 +78,11 @@ def _case3d():
 def test_diagonal_form_leaves_the_iterates_unchanged(which, precision, monkeypatch):
     """Template rows in the diagonal form (gather offsets per 1024-row block,
     one lane per row) sum the same entries in the same order as the plain
-    template rows (HFH_DIAG=0)."""
+    template rows (HFH_DIAG=0).  The V-cycle's fine-level residual keeps its
+    CSR launch in both runs (HFH_AMG_DIAG=0): in the diagonal form it sums
+    each row on one lane instead of several."""
     monkeypatch.setenv("HFH_KRYLOV_LANES", "1")
+    monkeypatch.setenv("HFH_AMG_DIAG", "0")
     if which == "cavity_pcg":
         monkeypatch.setenv("HFH_AMG", "0")
     runs = []
