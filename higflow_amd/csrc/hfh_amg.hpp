@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "hfh_krylov.hpp"
@@ -288,14 +289,19 @@ inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<
 }
 
 // ------------------------------------------------------------ V-cycle kernels
-constexpr int AL = 4;     // lanes per row in the V-cycle kernels
+// Lanes per row AL (1, 2 or 4) of the V-cycle kernels: chosen per level
+// and operator from its mean row length (amg_lanes).
 constexpr int ANT = 256;  // threads per workgroup
+inline int amg_lanes(long nnz, long rows) {
+    const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
+    return avg <= 3.0 ? 1 : avg <= 6.0 ? 2 : 4;
+}
 
 // ki is null for block-diagonal systems (the blocks stop one by one; the
 // cycle runs until the host sees every block done)
 __device__ inline bool amg_done(const int *ki) { return ki && ki[KI_DONE] != 0; }
 
-template <typename T>
+template <typename T, int AL>
 __device__ inline T arow(int rb, int re, const int *__restrict__ ci, const T *__restrict__ v, const T *__restrict__ x) {
     T s = 0;
     for (int k = rb + (int)(threadIdx.x % AL); k < re; k += AL) s += v[k] * x[ci[k]];
@@ -307,7 +313,7 @@ __device__ inline T arow(int rb, int re, const int *__restrict__ ci, const T *__
 // r = b - A (omega D^-1 b).  The fine level's b may be of another type TB
 // (an FP64 Krylov solve preconditioned by the FP32 cycle); the cycle
 // computes in T.
-template <typename T, typename TB = T>
+template <typename T, typename TB, int AL>
 __global__ void __launch_bounds__(ANT) k_amg_resid(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                    const T *__restrict__ v, const T *__restrict__ dinv, T omega,
                                                    const TB *__restrict__ b, T *__restrict__ r, const int *ki) {
@@ -364,14 +370,14 @@ __global__ void __launch_bounds__(KNT) k_amg_resid_d(int n, KRows<T> A, const T 
 }
 
 // b_c = R r
-template <typename T>
+template <typename T, int AL>
 __global__ void __launch_bounds__(ANT) k_amg_restrict(int nc, const int *__restrict__ rp, const int *__restrict__ ci,
                                                       const T *__restrict__ v, const T *__restrict__ r,
                                                       T *__restrict__ bc, const int *ki) {
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < nc;
-    const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, r);
+    const T s = arow<T, AL>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, r);
     if (in && threadIdx.x % AL == 0) bc[i] = s;
 }
 
@@ -418,7 +424,7 @@ __global__ void __launch_bounds__(ANT) k_amg_dense(int nc, const TC *__restrict_
 }
 
 // x = omega D^-1 b + P x_c
-template <typename T, typename TB = T>
+template <typename T, typename TB, int AL>
 __global__ void __launch_bounds__(ANT) k_amg_prolong(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                      const T *__restrict__ v, const T *__restrict__ dinv, T omega,
                                                      const TB *__restrict__ b, const T *__restrict__ xc,
@@ -426,12 +432,12 @@ __global__ void __launch_bounds__(ANT) k_amg_prolong(int n, const int *__restric
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < n;
-    const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, xc);
+    const T s = arow<T, AL>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, xc);
     if (in && threadIdx.x % AL == 0) x[i] = omega * (dinv[i] * (T)b[i]) + s;
 }
 
 // y = x + omega D^-1 (b - A x)
-template <typename T, typename TB = T, typename TY = T>
+template <typename T, typename TB, typename TY, int AL>
 __global__ void __launch_bounds__(ANT) k_amg_post(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const T *__restrict__ v, const T *__restrict__ dinv, T omega,
                                                   const TB *__restrict__ b, const T *__restrict__ x,
@@ -439,7 +445,7 @@ __global__ void __launch_bounds__(ANT) k_amg_post(int n, const int *__restrict__
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < n;
-    const T s = arow<T>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, x);
+    const T s = arow<T, AL>(in ? rp[i] : 0, in ? rp[i + 1] : 0, ci, v, x);
     if (in && threadIdx.x % AL == 0) y[i] = (TY)(x[i] + omega * (dinv[i] * ((T)b[i] - s)));
 }
 
@@ -447,7 +453,7 @@ __global__ void __launch_bounds__(ANT) k_amg_post(int n, const int *__restrict__
 // post-smoothing of one level in one launch.  r = b - A (omega D^-1 b) from
 // the level's residual launch, so for x = omega D^-1 b + P x_c the smoother's
 // b - A x is r - (A P) x_c, with A P stored at set-up.
-template <typename T, typename TB = T, typename TY = T>
+template <typename T, typename TB, typename TY, int AL>
 __global__ void __launch_bounds__(ANT) k_amg_prolong_post(int n, const int *__restrict__ prp,
                                                           const int *__restrict__ pci, const T *__restrict__ pv,
                                                           const int *__restrict__ qrp, const int *__restrict__ qci,
@@ -458,8 +464,8 @@ __global__ void __launch_bounds__(ANT) k_amg_prolong_post(int n, const int *__re
     if (amg_done(ki)) return;
     const int i = (blockIdx.x * ANT + threadIdx.x) / AL;
     const bool in = i < n;
-    const T s1 = arow<T>(in ? prp[i] : 0, in ? prp[i + 1] : 0, pci, pv, xc);
-    const T s2 = arow<T>(in ? qrp[i] : 0, in ? qrp[i + 1] : 0, qci, qv, xc);
+    const T s1 = arow<T, AL>(in ? prp[i] : 0, in ? prp[i + 1] : 0, pci, pv, xc);
+    const T s2 = arow<T, AL>(in ? qrp[i] : 0, in ? qrp[i + 1] : 0, qci, qv, xc);
     if (in && threadIdx.x % AL == 0) {
         const T di = dinv[i];
         y[i] = (TY)((omega * (di * (T)b[i]) + s1) + omega * (di * (r[i] - s2)));
@@ -546,6 +552,14 @@ class AmgPrecond {
                 upi(d.Qci, hl[l].AP.ci, st);
                 upt(d.Qv, hl[l].AP.v, st);
             }
+            d.la = amg_lanes((long)hl[l].A.v.size(), d.n);
+            d.lr = amg_lanes((long)hl[l].R.v.size(), d.nc);
+            d.lp = amg_lanes((long)hl[l].P.v.size(), d.n);
+            d.lq = amg_lanes((long)std::max(hl[l].P.v.size(), hl[l].AP.v.size()), d.n);
+            if (const char *e = std::getenv("HFH_AMG_LANES")) {
+                const int f = std::atoi(e);
+                if (f == 1 || f == 2 || f == 4) d.la = d.lr = d.lp = d.lq = f;
+            }
             d.r.alloc(d.n);
             d.x.alloc(d.n);
             d.bnext.alloc(d.nc);
@@ -587,6 +601,7 @@ class AmgPrecond {
   private:
     struct Dev {
         int n = 0, nc = 0;
+        int la = 4, lr = 4, lp = 4, lq = 4;  // lanes per row: A, R, P, and P with A P (amg_lanes)
         KBuf<int> Arp, Aci, Prp, Pci, Rrp, Rci, Qrp, Qci;  // Q = A P
         KBuf<T> Av, dinv, Pv, Rv, Qv, r, x, bnext, xnext;
     };
@@ -606,7 +621,7 @@ class AmgPrecond {
             throw std::runtime_error("hipMemcpyAsync failed");
         if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("hipStreamSynchronize failed");
     }
-    static int grid(int rows) { return std::max(1, (rows * AL + ANT - 1) / ANT); }
+    static int grid(int rows, int al) { return std::max(1, (rows * al + ANT - 1) / ANT); }
 
     // y = V(b) at level l (y, b of that level's size)
     template <typename TB, typename TY>
@@ -615,17 +630,20 @@ class AmgPrecond {
         if (l == 0 && A0.dw)
             k_amg_resid_d<T, TB><<<(d.n + KNT - 1) / KNT, KNT, 0, st>>>(d.n, A0, d.dinv.p, omega, b, d.r.p, ki);
         else
-            k_amg_resid<T, TB><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.r.p,
-                                                          ki);
+            with_lanes(d.la, [&](auto C) {
+                constexpr int AL = decltype(C)::value;
+                k_amg_resid<T, TB, AL><<<grid(d.n, AL), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega,
+                                                                      b, d.r.p, ki);
+            });
         if (l + 1 < nlev) {
-            k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
+            restrict_(d, ki, st);
             cycle<T, T>(l + 1, d.bnext.p, d.xnext.p, ki, st);
         } else if (ncoarse <= 256) {
             k_amg_coarse<T><<<1, 1024, ncoarse * sizeof(T), st>>>(ncoarse, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, cinv_.p,
                                                                    d.xnext.p, ki);
         } else {
             // larger coarsest level: restrict, then a wave per dense row
-            k_amg_restrict<T><<<grid(d.nc), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p, ki);
+            restrict_(d, ki, st);
             // the dense inverse is read in FP32 (half the bytes of the
             // coarsest solve; it is part of a preconditioner, and the Krylov
             // iteration around it keeps its precision)
@@ -633,15 +651,39 @@ class AmgPrecond {
                 ncoarse, cinv32_.p, d.bnext.p, d.xnext.p, ki);
         }
         if (fuse_pp) {
-            k_amg_prolong_post<T, TB, TY><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p,
-                                                                     d.Qv.p, d.dinv.p, omega, b, d.r.p, d.xnext.p,
-                                                                     y, ki);
+            with_lanes(d.lq, [&](auto C) {
+                constexpr int AL = decltype(C)::value;
+                k_amg_prolong_post<T, TB, TY, AL><<<grid(d.n, AL), ANT, 0, st>>>(
+                    d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p, d.Qv.p, d.dinv.p, omega, b, d.r.p, d.xnext.p, y, ki);
+            });
             return;
         }
-        k_amg_prolong<T, TB><<<grid(d.n), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b,
-                                                        d.xnext.p, d.x.p, ki);
-        k_amg_post<T, TB, TY><<<grid(d.n), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b, d.x.p, y,
-                                                         ki);
+        with_lanes(d.lp, [&](auto C) {
+            constexpr int AL = decltype(C)::value;
+            k_amg_prolong<T, TB, AL><<<grid(d.n, AL), ANT, 0, st>>>(d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.dinv.p, omega, b,
+                                                                    d.xnext.p, d.x.p, ki);
+        });
+        with_lanes(d.la, [&](auto C) {
+            constexpr int AL = decltype(C)::value;
+            k_amg_post<T, TB, TY, AL><<<grid(d.n, AL), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega, b,
+                                                                     d.x.p, y, ki);
+        });
+    }
+    void restrict_(Dev &d, const int *ki, hipStream_t st) {
+        with_lanes(d.lr, [&](auto C) {
+            constexpr int AL = decltype(C)::value;
+            k_amg_restrict<T, AL><<<grid(d.nc, AL), ANT, 0, st>>>(d.nc, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, d.bnext.p,
+                                                                  ki);
+        });
+    }
+    template <typename F>
+    static void with_lanes(int al, F &&f) {
+        if (al == 1)
+            f(std::integral_constant<int, 1>{});
+        else if (al == 2)
+            f(std::integral_constant<int, 2>{});
+        else
+            f(std::integral_constant<int, 4>{});
     }
 };
 
