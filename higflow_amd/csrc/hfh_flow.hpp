@@ -291,6 +291,7 @@ class Flow : public FlowBase {
         int nrow = 0, ncomp = 1, in_stride = 0, out_stride = 0;
         int lanes = 1;  // lanes per row (from the mean row length)
         bool has_y = false, f32 = false;
+        std::vector<int> order;  // device row k is the group's row order[k]
     } dDu_, dK_, dCv_, dMom_, dDiv_, dGrad_;
     // mixed form with HFH_MIXED_KERNELS=1: momentum, velocity-gradient and
     // constitutive evaluations in FP32 arithmetic (FP64 state, eigen-

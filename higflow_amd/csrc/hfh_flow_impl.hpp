@@ -198,6 +198,9 @@ void Flow<D>::upload_device() {
                 }
         }
         upload_group(dGrad_, gGrad_, 0, 0, true);
+        std::vector<int> yp(y.size());
+        for (size_t k = 0; k < yp.size(); k++) yp[k] = y[dGrad_.order[k]];
+        y.swap(yp);
         dGrad_.y.upload(y.data(), y.size(), st_);
         HFH_CHECK(hipStreamSynchronize(st_));  // y is a local
     }
@@ -319,57 +322,95 @@ void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool
     dg.out_stride = out_stride;
     dg.has_y = has_y;
     dg.f32 = f32;
+    dg.order.clear();
     if (dg.nrow == 0) return;
-    dg.rp.upload(g.rowptr.data(), g.rowptr.size(), st_);
-    std::vector<int> col = g.col;
+    // template rows: the operator bytes of repeated stencils shrink to a
+    // template id and a column base per row
+    std::vector<int> code, base, tcnt, toff;
+    std::vector<double> tw;
+    long gen_nnz = 0;
+    const long covered = group_templates(g, code, base, tcnt, toff, tw, gen_nnz);
+    if (verbose_)
+        std::fprintf(stderr, "hfh: group of %d rows (%zu entries): %ld template rows, %zu templates\n", dg.nrow,
+                     g.col.size(), covered, tcnt.size());
+    // device row order: the rows of each template together, in recording
+    // order (so with increasing column bases), then the general rows.  A
+    // wave then reads one template's entries and gathers neighbouring state
+    // values; each row still sums its own entries in its own order, and the
+    // rows write to their own output slots.  (HFH_GROUP_ORDER=0: recording
+    // order.)
+    dg.order.resize(dg.nrow);
+    {
+        const char *eo = std::getenv("HFH_GROUP_ORDER");
+        const bool sort = covered > 0 && !(eo && std::strcmp(eo, "0") == 0);
+        if (sort) {
+            const int nt = (int)tcnt.size();
+            std::vector<long> start(nt + 2, 0);
+            for (int r = 0; r < dg.nrow; r++) start[(code[r] >= 0 ? code[r] : nt) + 1]++;
+            for (int t = 0; t <= nt; t++) start[t + 1] += start[t];
+            for (int r = 0; r < dg.nrow; r++) dg.order[start[code[r] >= 0 ? code[r] : nt]++] = r;
+        } else {
+            for (int r = 0; r < dg.nrow; r++) dg.order[r] = r;
+        }
+    }
+    const std::vector<int> &ord = dg.order;
+    std::vector<int> rp(dg.nrow + 1, 0), col, outp(dg.nrow);
+    std::vector<double> val;
+    col.reserve(g.col.size());
+    val.reserve(g.val.size());
+    for (int k = 0; k < dg.nrow; k++) {
+        const int r = ord[k];
+        for (int q = g.rowptr[r]; q < g.rowptr[r + 1]; q++) {
+            col.push_back(g.col[q]);
+            val.push_back(g.val[q]);
+        }
+        rp[k + 1] = (int)col.size();
+        outp[k] = g.out[r];
+    }
     if (col.empty()) col.push_back(0);
-    std::vector<double> val = g.val;
     if (val.empty()) val.push_back(0.0);
+    dg.rp.upload(rp.data(), rp.size(), st_);
     dg.col.upload(col.data(), col.size(), st_);
+    std::vector<float> v32;
     if (f32) {
-        std::vector<float> v32(val.begin(), val.end());
+        v32.assign(val.begin(), val.end());
         dg.val32.upload(v32.data(), v32.size(), st_);
-        HFH_CHECK(hipStreamSynchronize(st_));
         dg.cst32.alloc(dg.nrow);
     } else {
         dg.val.upload(val.data(), val.size(), st_);
     }
-    dg.out.upload(g.out.data(), g.out.size(), st_);
+    dg.out.upload(outp.data(), outp.size(), st_);
     dg.cst.alloc(dg.nrow);
-    // template rows: the operator bytes of repeated stencils shrink to a
-    // template id and a column base per row
     dg.ntempl = 0;
     dg.op_bytes = (long)g.col.size() * (4 + (f32 ? 4 : 8)) + (long)(dg.nrow + 1) * 4;
     double avg = (double)g.col.size() / dg.nrow;
-    {
-        std::vector<int> code, base, tcnt, toff;
-        std::vector<double> tw;
-        long gen_nnz = 0;
-        const long covered = group_templates(g, code, base, tcnt, toff, tw, gen_nnz);
-        if (verbose_)
-            std::fprintf(stderr, "hfh: group of %d rows (%zu entries): %ld template rows, %zu templates\n", dg.nrow,
-                         g.col.size(), covered, tcnt.size());
-        if (covered > 0) {
-            dg.ntempl = (int)tcnt.size();
-            dg.tcode.upload(code.data(), code.size(), st_);
-            dg.tbase.upload(base.data(), base.size(), st_);
-            dg.tcnt.upload(tcnt.data(), tcnt.size(), st_);
-            dg.toff.upload(toff.data(), toff.size(), st_);
-            std::vector<float> tw32(tw.begin(), tw.end());
-            if (f32)
-                dg.tw32.upload(tw32.data(), tw32.size(), st_);
-            else
-                dg.tw.upload(tw.data(), tw.size(), st_);
-            HFH_CHECK(hipStreamSynchronize(st_));
-            const long gen_rows = dg.nrow - covered;
-            dg.op_bytes = (long)dg.nrow * 8 + gen_nnz * (4 + (f32 ? 4 : 8)) + gen_rows * 8 +
-                          (long)dg.ntempl * (4 + GT_MAX * (4 + (f32 ? 4 : 8)));
-            // lanes by the entries a row actually reads
-            long entries = gen_nnz;
-            for (int r = 0; r < dg.nrow; r++)
-                if (code[r] >= 0) entries += tcnt[code[r]];
-            avg = (double)entries / dg.nrow;
+    std::vector<int> pcode, pbase;
+    std::vector<float> tw32;
+    if (covered > 0) {
+        dg.ntempl = (int)tcnt.size();
+        pcode.resize(dg.nrow);
+        pbase.resize(dg.nrow);
+        for (int k = 0; k < dg.nrow; k++) {
+            pcode[k] = code[ord[k]];
+            pbase[k] = base[ord[k]];
         }
+        dg.tcode.upload(pcode.data(), pcode.size(), st_);
+        dg.tbase.upload(pbase.data(), pbase.size(), st_);
+        dg.tcnt.upload(tcnt.data(), tcnt.size(), st_);
+        dg.toff.upload(toff.data(), toff.size(), st_);
+        tw32.assign(tw.begin(), tw.end());
+        if (f32)
+            dg.tw32.upload(tw32.data(), tw32.size(), st_);
+        else
+            dg.tw.upload(tw.data(), tw.size(), st_);
+        const long gen_rows = dg.nrow - covered;
+        dg.op_bytes = (long)dg.nrow * 8 + gen_nnz * (4 + (f32 ? 4 : 8)) + gen_rows * 8 +
+                      (long)dg.ntempl * (4 + GT_MAX * (4 + (f32 ? 4 : 8)));
+        // lanes by the entries a row actually reads
+        long entries = gen_nnz;
+        for (int r = 0; r < dg.nrow; r++)
+            if (code[r] >= 0) entries += tcnt[code[r]];
+        avg = (double)entries / dg.nrow;
     }
     HFH_CHECK(hipStreamSynchronize(st_));  // the host copies above are locals
     // lanes per row from the mean row length (HFH_INTERP_LANES overrides)
@@ -495,7 +536,13 @@ void Flow<D>::refold_constants() {
     std::vector<std::vector<float>> keep32;
     for (auto &p : gs) {
         if (p.d->nrow == 0) continue;
-        keep.push_back(p.g->constants(bcval_));
+        {
+            // in the device row order (upload_group)
+            const std::vector<double> c0 = p.g->constants(bcval_);
+            std::vector<double> c(c0.size());
+            for (size_t k = 0; k < c.size(); k++) c[k] = c0[p.d->order[k]];
+            keep.push_back(std::move(c));
+        }
         const std::vector<double> &c = keep.back();
         if (p.d->f32) {
             keep32.emplace_back(c.begin(), c.end());
