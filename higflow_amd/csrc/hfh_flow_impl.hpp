@@ -340,9 +340,11 @@ void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool
     // rows write to their own output slots.  (HFH_GROUP_ORDER=0: recording
     // order.)
     dg.order.resize(dg.nrow);
+    bool grouped = false;
     {
         const char *eo = std::getenv("HFH_GROUP_ORDER");
         const bool sort = covered > 0 && !(eo && std::strcmp(eo, "0") == 0);
+        grouped = sort;
         if (sort) {
             const int nt = (int)tcnt.size();
             std::vector<long> start(nt + 2, 0);
@@ -413,8 +415,13 @@ void Flow<D>::upload_group(DGroup &dg, const Group &g, int, int out_stride, bool
         avg = (double)entries / dg.nrow;
     }
     HFH_CHECK(hipStreamSynchronize(st_));  // the host copies above are locals
-    // lanes per row from the mean row length (HFH_INTERP_LANES overrides)
-    dg.lanes = avg < 4 ? 1 : avg < 8 ? 2 : avg < 16 ? 4 : avg < 32 ? 8 : 16;
+    // lanes per row from the mean row length (HFH_INTERP_LANES overrides).
+    // Rows grouped by template gather neighbouring values across the rows
+    // of a wave, so fewer lanes per row (more rows per wave) pay there:
+    // momentum rows of about 15 entries take 790 us on 2 lanes against 916
+    // on 4 (the 64-cavity sweep)
+    dg.lanes = grouped ? (avg < 4 ? 1 : avg < 32 ? 2 : avg < 64 ? 4 : 8)
+                       : (avg < 4 ? 1 : avg < 8 ? 2 : avg < 16 ? 4 : avg < 32 ? 8 : 16);
     if (const char *e = std::getenv("HFH_INTERP_LANES")) {
         int f = std::atoi(e);
         if (f == 1 || f == 2 || f == 4 || f == 8 || f == 16) dg.lanes = f;
