@@ -57,7 +57,8 @@ WORKLOADS = {
     # [0.1, 2]; 1,225,216 cells), beta=0.5, dt=5e-4, semi-implicit Euler,
     # all stepped together (one launch per stage)
     "sweep64_ve_cavity": dict(kind="sweep", count=64, seed=0, dt=5e-4, steps=100,
-                              roof=(12, "k_kcg_spmv<float> (velocity PCG SpMV, mixed form)", "mixed")),
+                              roof=(13, "k_amg_prolong_post<float> (velocity V-cycle, fine level, mixed form)",
+                                    "mixed")),
 }
 DEFAULT = "sweep64_ve_cavity"
 # launch A of the fused FP64 BiCGStab on the semi-implicit velocity block
@@ -228,11 +229,10 @@ def main():
     stmx = stage_pass(simmx, k)
     conv, iters, resid, _ = sim64.last_solve()
 
-    # dominant kernel (rocprofv3 kernel trace of this workload): launch A of
-    # the fused FP64 BiCGStab on the semi-implicit velocity block system
-    # (x, r, p updates and v = (A D^-1) p in one launch); launch B is the
-    # second launch of the same iteration.  Explicit workloads probe the
-    # pressure system's launch A instead.
+    # the dominant kernel of the workload's kernel trace (its `roof` entry;
+    # by default launch A of the fused FP64 BiCGStab on the semi-implicit
+    # velocity block system, or the pressure system's for explicit runs),
+    # timed live with hipEvents on the library stream
     roof = None
     rid, rname, rform = spec.get("roof", DEFAULT_ROOF)
     pr = probe(simmx if rform == "mixed" else sim64, rid, 400)
@@ -249,6 +249,7 @@ def main():
     kernels = {}
     for which, name, sim in ((9, "k_kbcg_a<double>", sim64), (11, "k_kcg_spmv<double>(velocity)", sim64),
                              (12, "k_kcg_spmv<float>(velocity)", simmx),
+                             (13, "k_amg_prolong_post<float>(velocity, fine level)", simmx),
                              (10, "k_kbcg_b<double>", sim64), (6, "k_kbcg_b<double>(pressure)", sim64),
                              (7, "k_kbcg_b<float>(pressure)", simmx), (0, "k_spmv_dot<double>", sim64),
                              (1, "k_momentum", sim64), (2, "k_interp(momentum rows)", sim64),

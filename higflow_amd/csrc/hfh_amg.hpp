@@ -588,6 +588,40 @@ class AmgPrecond {
         cycle<TB, TY>(0, b, y, ki, st);
     }
 
+    // Times `reps` launches of the fine level's prolongation and
+    // post-smoothing (k_amg_prolong_post) on the level's current vectors,
+    // writing a buffer of its own; returns the average ms (negative without
+    // a fused hierarchy) and the algorithmic bytes of one launch: the P and
+    // A P rows as stored (two row pointers each), b, r, dinv read and y
+    // written once per row, the coarse correction once.
+    float probe_pp(int reps, long *bytes, hipStream_t st) {
+        if (!ready || nlev == 0 || !fuse_pp) return -1.0f;
+        Dev &d = lv_[0];
+        probe_y_.alloc(d.n);
+        k_fill<T><<<nblocks(d.n), BLOCK, 0, st>>>(d.n, d.x.p, T(0));
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            throw std::runtime_error("hipEventCreate failed");
+        (void)hipEventRecord(a, st);
+        with_lanes(d.lq, [&](auto C) {
+            constexpr int AL = decltype(C)::value;
+            for (int q = 0; q < reps; q++)
+                k_amg_prolong_post<T, T, T, AL><<<grid(d.n, AL), ANT, 0, st>>>(
+                    d.n, d.Prp.p, d.Pci.p, d.Pv.p, d.Qrp.p, d.Qci.p, d.Qv.p, d.dinv.p, omega, d.x.p, d.r.p,
+                    d.xnext.p, probe_y_.p, nullptr);
+        });
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *bytes = (long)(d.Pv.n + d.Qv.n) * (4 + (long)sizeof(T)) + 2L * (d.n + 1) * 4 + (long)d.n * 4 * sizeof(T) +
+                 (long)d.nc * sizeof(T);
+        return reps > 0 ? ms / reps : 0.0f;
+    }
+    int fine_lanes_pp() const { return lv_.empty() ? 0 : lv_[0].lq; }
+
     long bytes_per_cycle() const {
         long by = 0;
         for (const Dev &d : lv_) {
@@ -608,6 +642,7 @@ class AmgPrecond {
     std::vector<Dev> lv_;
     KBuf<T> cinv_;
     KBuf<float> cinv32_;  // the dense coarsest inverse as read by k_amg_dense
+    KBuf<T> probe_y_;
 
     static void upi(KBuf<int> &d, const std::vector<int> &h, hipStream_t st) {
         d.alloc(h.size());

@@ -163,12 +163,10 @@ void Flow<D>::upload_device() {
     // batch (between member blocks) are never written and must read zero
     k_fill<double><<<nblocks(Nc_), BLOCK, 0, st_>>>(Nc_, bvec_.p, 0.0);
     for (int d = 0; d < D; d++) {
-        dMomSlots_[d].upload(momSlots_[d].data(), momSlots_[d].size(), st_);
         dMomFlags_[d].upload(momFlags_[d].data(), momFlags_[d].size(), st_);
         dMomDelta_[d].upload(momDelta_[d].data(), momDelta_[d].size(), st_);
     }
     if (!veSlots_.empty()) {
-        dVeSlots_.upload(veSlots_.data(), veSlots_.size(), st_);
         dVeFlags_.upload(veFlags_.data(), veFlags_.size(), st_);
         dVeDelta_.upload(veDelta_.data(), veDelta_.size(), st_);
     }
@@ -186,6 +184,36 @@ void Flow<D>::upload_device() {
     upload_group(dCv_, gCv_, 0, 0, false, k32_);
     upload_group(dMom_, gMom_, 0, 0, false, k32_);
     upload_group(dDiv_, gDiv_, 0, 0, false);
+    // The point-value regions (momentum points, K at CUBISTA points,
+    // cell-centre velocities) are laid out in the device row order of their
+    // groups, so each group writes its region contiguously; the slot tables
+    // of the momentum and constitutive kernels are relabelled to match
+    // (rows of one template hold one point kind of consecutive elements, so
+    // a wave's slot reads stay contiguous too).
+    {
+        std::vector<int> ms[D], vs = veSlots_;
+        for (int d = 0; d < D; d++) ms[d] = momSlots_[d];
+        auto relabel = [&](DGroup &dg, int oP, std::vector<int> *tabs, int ntab) {
+            const int n = dg.nrow;
+            if (n == 0 || (int)dg.order.size() != n) return;
+            std::vector<int> inv(n), outp(n);
+            for (int k = 0; k < n; k++) {
+                inv[dg.order[k]] = k;
+                outp[k] = oP + k;
+            }
+            for (int q = 0; q < ntab; q++)
+                for (int &v : tabs[q])
+                    if (v >= oP && v < oP + n) v = oP + inv[v - oP];
+            dg.out.upload(outp.data(), outp.size(), st_);
+            HFH_CHECK(hipStreamSynchronize(st_));  // outp is a local
+        };
+        relabel(dMom_, oPmom_, ms, D);
+        relabel(dK_, oPK_, &vs, 1);
+        relabel(dCv_, oPcv_, &vs, 1);
+        for (int d = 0; d < D; d++) dMomSlots_[d].upload(ms[d].data(), ms[d].size(), st_);
+        if (!vs.empty()) dVeSlots_.upload(vs.data(), vs.size(), st_);
+        HFH_CHECK(hipStreamSynchronize(st_));  // the tables are locals
+    }
     // projection rows: u = us - dt * grad
     {
         std::vector<int> y(gGrad_.nrows());
@@ -927,6 +955,19 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
             (void)hipEventDestroy(b);
             if (!hbatched_) return -1;
             float ms = which == 11 ? hsysB_.k64.probe_cg(reps, &by, st_) : hsysB_.k32.probe_cg(reps, &by, st_);
+            *avg_ms = ms;
+            *bytes = by;
+            return 0;
+        }
+        case 13: {
+            // fine-level prolongation + post-smoothing launch of the FP32
+            // V-cycle of the semi-implicit velocity block system (AMG-PCG of
+            // the symmetric velocity systems of uniform meshes)
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (!hbatched_ || !hsysB_.k32.amg_on()) return -1;
+            const float ms = hsysB_.k32.amg.probe_pp(reps, &by, st_);
+            if (ms < 0) return -1;
             *avg_ms = ms;
             *bytes = by;
             return 0;

@@ -216,8 +216,10 @@ int hfh_set_timing(hfh_solver *s, int on);
  * 9 / 10 the fused FP64 BiCGStab launch A (update + SpMV) / launch B (SpMV)
  * of the semi-implicit velocity block system (the pressure system's when
  * the step has none), 11 / 12 the fused PCG SpMV launch of that velocity
- * system in FP64 / FP32.  Average ms per launch and the algorithmic bytes
- * of one launch.  Call between steps. */
+ * system in FP64 / FP32, 13 the fine-level prolongation + post-smoothing
+ * launch of that system's FP32 V-cycle (AMG-PCG velocity systems).
+ * Average ms per launch and the algorithmic bytes of one launch; -1 when
+ * the step has no such launch.  Call between steps. */
 int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
 int hfh_sync(hfh_solver *s);
 /* the assembled systems as CSR: which 0 pressure (the matrix slv_set_Ai
