@@ -466,6 +466,7 @@ class DeviceSystem {
     DeviceSystem(const DeviceSystem &) = delete;
     DeviceSystem &operator=(const DeviceSystem &) = delete;
     ~DeviceSystem() {
+        clear_graphs();
         if (hpin_) (void)hipHostFree(hpin_);
         for (auto &e : tev_)
             if (e) (void)hipEventDestroy(e);
@@ -508,7 +509,16 @@ class DeviceSystem {
         seg_off = segs.empty() ? std::vector<int>{0, n} : segs;
         off_ks_ = ((nseg * KI_N * (int)sizeof(int) + 63) / 64) * 64;
         off_sc_ = off_ks_ + ((nseg * KS_N * (int)sizeof(double) + 63) / 64) * 64;
-        const size_t pin = (size_t)off_sc_ + 256;
+        off_rr_ = off_sc_ + 256;
+        const size_t pin = (size_t)off_rr_ + 2 * (size_t)nb * sizeof(double);
+        clear_graphs();
+        {
+            // off by default: on the 64-cavity sweep the graph took the FP64
+            // step from 8.87 to 11.84 ms (a launch per node with its own
+            // dispatch wait, plus the budget's launches past convergence)
+            const char *eg = std::getenv("HFH_GRAPH");
+            graph_on_ = eg && std::strcmp(eg, "1") == 0;
+        }
         if (hpin_ && pin > hpin_size_) {
             (void)hipHostFree(hpin_);
             hpin_ = nullptr;
@@ -567,9 +577,19 @@ class DeviceSystem {
     // relative residual.
     template <typename T>
     void read_status(const FusedKrylov<T> &k, int *hi, double *hs, hipStream_t st) {
+        copy_status<T>(k, st);
+        HFH_CHECK(hipStreamSynchronize(st));
+        parse_status<T>(hi, hs);
+    }
+    // flags and scalars to pinned memory (no wait; capturable)
+    template <typename T>
+    void copy_status(const FusedKrylov<T> &k, hipStream_t st) {
         HFH_CHECK(hipMemcpyAsync(hpin_, k.ki.p, (size_t)nseg * KI_N * sizeof(int), hipMemcpyDeviceToHost, st));
         HFH_CHECK(hipMemcpyAsync(hpin_ + off_ks_, k.ks.p, (size_t)nseg * KS_N * sizeof(T), hipMemcpyDeviceToHost, st));
-        HFH_CHECK(hipStreamSynchronize(st));
+    }
+    // reads what copy_status left in pinned memory (after a wait)
+    template <typename T>
+    void parse_status(int *hi, double *hs) {
         seg_hi_.resize((size_t)nseg * KI_N);
         seg_hs_.resize((size_t)nseg * KS_N);
         std::memcpy(seg_hi_.data(), hpin_, (size_t)nseg * KI_N * sizeof(int));
@@ -625,13 +645,17 @@ class DeviceSystem {
         for (int restart = 0; restart <= max_restarts; restart++) {
             if (restart > 0)  // keep the current x in case the restart does worse
                 HFH_CHECK(hipMemcpyAsync(xbak.p, x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-            k64.init<double>(spd, b, x, st);
-            S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
-            // true residual per segment (segments start on 256-row blocks)
-            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
-            HFH_CHECK(hipMemcpyAsync(rr.data(), part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
-            HFH_CHECK(hipMemcpyAsync(bb.data(), part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
-            HFH_CHECK(hipStreamSynchronize(st));
+            if (restart == 0 && graph_on_) {
+                S.iters += run_graph64(spd, b, x, rtol, atol, maxit, hi, hs, rr, bb, st);
+            } else {
+                k64.init<double>(spd, b, x, st);
+                S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
+                // true residual per segment (segments start on 256-row blocks)
+                k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+                HFH_CHECK(hipMemcpyAsync(rr.data(), part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+                HFH_CHECK(hipMemcpyAsync(bb.data(), part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+                HFH_CHECK(hipStreamSynchronize(st));
+            }
             bool all_conv = true, worth = false;
             double worst = 0;
             std::vector<SolveStats> gs(nseg);
@@ -665,6 +689,71 @@ class DeviceSystem {
         }
         for (auto &g : seg_stats) g.restarts = S.restarts;
         return S;
+    }
+
+    // The first pass of an FP64 fused solve as one captured hipGraph: the
+    // initialisation, a budget of iterations (the last solve's count plus a
+    // quarter, rounded up to 8; launches after the stopping test return at
+    // once), the true-residual partials and the copies of the flags and
+    // partials to pinned memory.  The host waits once.  A solve that needs
+    // more iterations continues with host-checked chunks from where the
+    // graph stopped, so the iterates are those of the host-checked loop.
+    // One graph per (matrix form, vectors, budget, tolerances), kept until
+    // the matrix is set again.  HFH_GRAPH=1 turns it on; the default is the
+    // host-checked loop, which measured faster.
+    int run_graph64(bool spd, const double *b, double *x, double rtol, double atol, int maxit, int *hi, double *hs,
+                    std::vector<double> &rr, std::vector<double> &bb, hipStream_t st) {
+        const int want = k64.pred + 2 + k64.pred / 4;
+        const int budget = std::min(maxit + 1, std::max(8, (want + 7) / 8 * 8));
+        hipGraphExec_t exec = nullptr;
+        for (auto &g : graphs_)
+            if (g.spd == spd && g.a32 == k64.use_amg32 && g.b == b && g.x == x && g.budget == budget &&
+                g.rtol == rtol && g.atol == atol && g.maxit == maxit)
+                exec = g.exec;
+        double *prr = reinterpret_cast<double *>(hpin_ + off_rr_);
+        if (!exec) {
+            if (graphs_.size() >= 16) clear_graphs();
+            hipGraph_t graph = nullptr;
+            HFH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            k64.init<double>(spd, b, x, st);
+            k64.iterate(spd, 0, budget, maxit, rtol, atol, x, st);
+            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            copy_status<double>(k64, st);
+            HFH_CHECK(hipMemcpyAsync(prr, part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipMemcpyAsync(prr + nb, part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamEndCapture(st, &graph));
+            HFH_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            graphs_.push_back(GraphEntry{spd, k64.use_amg32, b, x, budget, rtol, atol, maxit, exec});
+        }
+        HFH_CHECK(hipGraphLaunch(exec, st));
+        HFH_CHECK(hipStreamSynchronize(st));
+        parse_status<double>(hi, hs);
+        int it = budget;
+        if (!hi[KI_DONE] && it <= maxit) {
+            // more iterations than the budget: host-checked chunks
+            for (;;) {
+                const int chunk = std::min(check_every, maxit + 1 - it);
+                k64.iterate(spd, it, chunk, maxit, rtol, atol, x, st);
+                it += chunk;
+                read_status<double>(k64, hi, hs, st);
+                if (hi[KI_DONE] || it > maxit) break;
+            }
+            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            HFH_CHECK(hipMemcpyAsync(prr, part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipMemcpyAsync(prr + nb, part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+        }
+        std::memcpy(rr.data(), prr, nb * sizeof(double));
+        std::memcpy(bb.data(), prr + nb, nb * sizeof(double));
+        const int used = hi[KI_DONE] ? hi[KI_ITERS] : it;
+        k64.pred = used;
+        return used;
+    }
+    void clear_graphs() {
+        for (auto &g : graphs_)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graphs_.clear();
     }
 
     // Mixed form: per system, FP32 refinement or the FP64 solve, whichever
@@ -1029,6 +1118,18 @@ class DeviceSystem {
 
   private:
     char *hpin_ = nullptr;  // pinned status: flags at 0, fused scalars at 64, outer scalars at 256
+    int off_rr_ = 512;      // pinned true-residual partials (r.r then b.b, nb each)
+    bool graph_on_ = false;  // first pass of FP64 fused solves as a captured graph (HFH_GRAPH=1)
+    struct GraphEntry {
+        bool spd, a32;
+        const double *b;
+        double *x;
+        int budget;
+        double rtol, atol;
+        int maxit;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs_;
 };
 
 }  // namespace hfh
