@@ -387,6 +387,69 @@ __global__ void __launch_bounds__(4 * BLOCK) k_residual_l4(int n, const int *__r
     }
 }
 
+// r = b - A x with the matrix in the diagonal form of the Krylov kernels
+// (KRows::dblk; rows without a template read CSR), one row per thread in
+// 1024-row workgroups; partials r.r and b.b per 256-row block as
+// k_residual_l4 writes them (one per BLOCK rows, nb of them).
+template <typename T>
+__global__ void __launch_bounds__(KNT) k_residual_d(int n, KRows<T> A, const T *__restrict__ x,
+                                                    const T *__restrict__ b, T *__restrict__ r, T *__restrict__ prr,
+                                                    T *__restrict__ pbb, int nb) {
+    const int i = blockIdx.x * KNT + (int)threadIdx.x;
+    const bool in = i < n;
+    const int *db = A.dblk + (size_t)blockIdx.x * (KD_MAX + 1);
+    const int nd = db[KD_MAX];
+    T xv[KD_MAX];
+#pragma unroll
+    for (int e = 0; e < KD_MAX; e++) {
+        int j = i + db[e];
+        j = j < 0 ? 0 : (j >= n ? n - 1 : j);
+        xv[e] = (in && e < nd) ? x[j] : T(0);
+    }
+    const int t = in ? A.code[i] : -1;
+    T s = 0;
+    if (t >= 0) {
+        const unsigned msk = A.dmask[t];
+        const T *w = A.dw + (size_t)t * KD_MAX;
+#pragma unroll
+        for (int e = 0; e < KD_MAX; e++) {
+            const T a = w[e];
+            if ((msk >> e) & 1u) s += a * xv[e];
+        }
+    } else if (in) {
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) s += A.val[k] * x[A.ci[k]];
+    }
+    T rr = 0, bb = 0;
+    if (in) {
+        const T bi = b[i], ri = bi - s;
+        r[i] = ri;
+        rr = ri * ri;
+        bb = bi * bi;
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    __shared__ T srr[KNT / 64], sbb[KNT / 64];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        srr[w] = rr;
+        sbb[w] = bb;
+    }
+    __syncthreads();
+    constexpr int WPB = BLOCK / 64;  // waves per 256-row block
+    if (threadIdx.x < KNT / BLOCK) {
+        const int q = threadIdx.x, pb = blockIdx.x * (KNT / BLOCK) + q;
+        T a = 0, c = 0;
+        for (int k = 0; k < WPB; k++) {
+            a += srr[q * WPB + k];
+            c += sbb[q * WPB + k];
+        }
+        if (pb < nb) {
+            prr[pb] = a;
+            pbb[pb] = c;
+        }
+    }
+}
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -518,6 +581,8 @@ class DeviceSystem {
             // dispatch wait, plus the budget's launches past convergence)
             const char *eg = std::getenv("HFH_GRAPH");
             graph_on_ = eg && std::strcmp(eg, "1") == 0;
+            const char *er = std::getenv("HFH_RESID_DIAG");
+            resid_diag_ = !(er && std::strcmp(er, "0") == 0);
         }
         if (hpin_ && pin > hpin_size_) {
             (void)hipHostFree(hpin_);
@@ -632,6 +697,16 @@ class DeviceSystem {
         return used;
     }
 
+    // r = b - A x and the per-256-row partials of r.r and b.b in part1 /
+    // part2: the diagonal-form rows when the FP64 solver has them
+    // (HFH_RESID_DIAG=0: the CSR launch)
+    void true_residual(const double *x, const double *b, hipStream_t st) {
+        if (k64.Av.dw && resid_diag_)
+            k_residual_d<double><<<(n + KNT - 1) / KNT, KNT, 0, st>>>(n, k64.Av, x, b, r.p, part1.p, part2.p, nb);
+        else
+            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+    }
+
     // FP64 fused solve: Jacobi-PCG (spd) or BiCGStab, warm-started from x.
     // The stopping test runs on the recursively updated residual; once it
     // passes, the true residual b - A x is checked and the iteration
@@ -651,7 +726,7 @@ class DeviceSystem {
                 k64.init<double>(spd, b, x, st);
                 S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
                 // true residual per segment (segments start on 256-row blocks)
-                k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+                true_residual(x, b, st);
                 HFH_CHECK(hipMemcpyAsync(rr.data(), part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
                 HFH_CHECK(hipMemcpyAsync(bb.data(), part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
                 HFH_CHECK(hipStreamSynchronize(st));
@@ -717,7 +792,7 @@ class DeviceSystem {
             HFH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
             k64.init<double>(spd, b, x, st);
             k64.iterate(spd, 0, budget, maxit, rtol, atol, x, st);
-            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            true_residual(x, b, st);
             copy_status<double>(k64, st);
             HFH_CHECK(hipMemcpyAsync(prr, part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipMemcpyAsync(prr + nb, part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -739,7 +814,7 @@ class DeviceSystem {
                 read_status<double>(k64, hi, hs, st);
                 if (hi[KI_DONE] || it > maxit) break;
             }
-            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            true_residual(x, b, st);
             HFH_CHECK(hipMemcpyAsync(prr, part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipMemcpyAsync(prr + nb, part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             HFH_CHECK(hipStreamSynchronize(st));
@@ -869,7 +944,7 @@ class DeviceSystem {
         std::vector<double> prr(nseg > 1 ? nb : 0), pbb(nseg > 1 ? nb : 0);
         std::vector<SolveStats> gs(nseg);
         for (;;) {
-            k_residual_l4<double><<<nb, 4 * BLOCK, 0, st>>>(n, rp.p, ci.p, v.p, x, b, r.p, part1.p, part2.p);
+            true_residual(x, b, st);
             if (nseg == 1) {
                 k_reduce<double><<<1, BLOCK, 0, st>>>(nb, part1.p, part2.p, sc.p, S_RR, S_BB);
                 HFH_CHECK(hipMemcpyAsync(hpin_ + off_sc_, sc.p, sizeof h, hipMemcpyDeviceToHost, st));
@@ -1120,6 +1195,7 @@ class DeviceSystem {
     char *hpin_ = nullptr;  // pinned status: flags at 0, fused scalars at 64, outer scalars at 256
     int off_rr_ = 512;      // pinned true-residual partials (r.r then b.b, nb each)
     bool graph_on_ = false;  // first pass of FP64 fused solves as a captured graph (HFH_GRAPH=1)
+    bool resid_diag_ = true;  // true residuals through the diagonal-form rows (HFH_RESID_DIAG)
     struct GraphEntry {
         bool spd, a32;
         const double *b;

@@ -78,11 +78,13 @@ def _case3d():
 def test_diagonal_form_leaves_the_iterates_unchanged(which, precision, monkeypatch):
     """Template rows in the diagonal form (gather offsets per 1024-row block,
     one lane per row) sum the same entries in the same order as the plain
-    template rows (HFH_DIAG=0).  The V-cycle's fine-level residual keeps its
-    CSR launch in both runs (HFH_AMG_DIAG=0): in the diagonal form it sums
-    each row on one lane instead of several."""
+    template rows (HFH_DIAG=0).  The V-cycle's fine-level residual and the
+    true residual of the solves keep their CSR launches in both runs
+    (HFH_AMG_DIAG=0, HFH_RESID_DIAG=0): in the diagonal form they sum each
+    row on one lane instead of several."""
     monkeypatch.setenv("HFH_KRYLOV_LANES", "1")
     monkeypatch.setenv("HFH_AMG_DIAG", "0")
+    monkeypatch.setenv("HFH_RESID_DIAG", "0")
     if which == "cavity_pcg":
         monkeypatch.setenv("HFH_AMG", "0")
     runs = []
