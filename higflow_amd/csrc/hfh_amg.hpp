@@ -297,9 +297,9 @@ inline int amg_lanes(long nnz, long rows) {
     return avg <= 3.0 ? 1 : avg <= 6.0 ? 2 : 4;
 }
 
-// ki is null for block-diagonal systems (the blocks stop one by one; the
-// cycle runs until the host sees every block done)
-__device__ inline bool amg_done(const int *ki) { return ki && ki[KI_DONE] != 0; }
+// `nd` is the solve's {stopped segments, segments} pair (KSeg::ndone): the
+// cycle's launches return at once when every segment has stopped
+__device__ inline bool amg_done(const int *nd) { return nd && nd[0] >= nd[1]; }
 
 template <typename T, int AL>
 __device__ inline T arow(int rb, int re, const int *__restrict__ ci, const T *__restrict__ v, const T *__restrict__ x) {
@@ -580,7 +580,8 @@ class AmgPrecond {
         return true;
     }
 
-    // y = V(b) on the fine level; `ki` is the solve's flag array.  b and y
+    // y = V(b) on the fine level; `ki` is the solve's {stopped segments,
+    // segments} pair (KSeg::ndone) or null.  b and y
     // may be of another precision than the cycle (TB, TY): the fine
     // level's first and last launches convert.
     template <typename TB, typename TY>
@@ -822,7 +823,7 @@ __global__ void __launch_bounds__(KNT) k_abcg_b(int n, int it, int maxit, T rtol
     const T rhi = in ? rh[i] : T(0);
     kblock_sync();
     if (pv[2] != T(0)) return;
-    if (krylov_stop<T>(c.lead, it == 0, it, maxit, pv[0], pv[1], rtol, atol, false, ks, ki)) return;
+    if (krylov_stop<T>(c.lead, it == 0, it, maxit, pv[0], pv[1], rtol, atol, false, ks, ki, S.ndone)) return;
     T acc[1] = {0};
     if (in && klane<L>() == 0) {
         v[i] = sv[0];

@@ -44,6 +44,7 @@ class FusedKrylov {
     int nseg = 1;
     std::vector<int> seg_off;
     KBuf<int> d_wseg, d_wbeg;
+    KBuf<int> ndone_;  // {stopped segments, segments} (KSeg::ndone)
     KSeg S;  // device view; S.wseg == nullptr for one segment
     // template rows (see KRows): one id per row, offsets shared by A and
     // A D^-1, one weight table each
@@ -132,6 +133,14 @@ class FusedKrylov {
     // KNT / L rows; segment offsets are multiples of KNT rows)
     void set_segments(int L, hipStream_t st) {
         S = KSeg();
+        {
+            ndone_.alloc(2);
+            const int h[2] = {0, nseg};
+            if (hipMemcpyAsync(ndone_.p, h, sizeof h, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                throw std::runtime_error("segment upload failed");
+            S.ndone = ndone_.p;
+        }
         if (nseg <= 1) return;
         const int rpw = KNT / L;
         std::vector<int> ws(nb), wb(nseg + 1);
@@ -511,7 +520,7 @@ class FusedKrylov {
     void launch_iter(bool cg, int it0, int count, int maxit, T rtol, T atol, T *x, hipStream_t st) {
         if (amg.ready && cg) {
             // AMG-PCG: z = V(r), rho = r.z, then the two PCG launches
-            const int *kc = nseg == 1 ? ki.p : nullptr;
+            const int *kc = S.ndone;
             for (int it = it0; it < it0 + count; it++) {
                 T *pold = (it & 1) ? p.p : p2.p, *pnew = (it & 1) ? p2.p : p.p;
                 precondition(r.p, z.p, kc, st);
@@ -524,9 +533,8 @@ class FusedKrylov {
             return;
         }
         if (amg.ready) {
-            // one segment: the cycle stops with the solve; several: it runs
-            // until the host sees every segment done
-            const int *kc = nseg == 1 ? ki.p : nullptr;
+            // the cycle's launches return at once when every segment has stopped
+            const int *kc = S.ndone;
             for (int it = it0; it < it0 + count; it++) {
                 k_abcg_a<T, L><<<nb, KNT, 0, st>>>(n, it, x, r.p, s.p, t.p, ph.p, sh.p, p.p, v.p, part.p, nb, ks.p,
                                                    ki.p, S);

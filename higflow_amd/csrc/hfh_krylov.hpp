@@ -75,6 +75,9 @@ enum { KP_RZ = 0, KP_RR, KP_BB, KP_PQ, KP_RV, KP_TS, KP_TT, KP_RS, KP_RT, KP_N }
 struct KSeg {
     const int *wseg = nullptr;
     const int *wbeg = nullptr;
+    // stopped segments and the segment count: {count, nseg}; the V-cycle
+    // skips its launches once every segment has stopped
+    int *ndone = nullptr;
 };
 struct KCtx {
     int w0, w1;  // the segment's workgroups
@@ -182,7 +185,7 @@ __device__ inline void write_partials(T (&v)[K], T *part, int nb, const int (&sl
 // (||r|| above 1e5 max(||r0||, ||b||), PETSc's default dtol).
 template <typename T>
 __device__ inline bool krylov_stop(bool lead, bool first, int it, int maxit, T rr, T bb, T rtol, T atol, bool breakdown,
-                                   T *ks, int *ki) {
+                                   T *ks, int *ki, int *ndone = nullptr) {
     T tol, rr0;
     if (first) {
         tol = rtol * sqrt(bb);
@@ -207,6 +210,7 @@ __device__ inline bool krylov_stop(bool lead, bool first, int it, int maxit, T r
             ki[KI_ITERS] = it;
             ki[KI_CONV] = conv ? 1 : 0;
             ki[KI_DONE] = 1;
+            if (ndone) atomicAdd(ndone, 1);
         }
     }
     return stop;
@@ -441,6 +445,7 @@ __global__ void __launch_bounds__(KNT / R) k_kinit(int n, int cg, int xzero, KRo
     }
     const int sl[3] = {KP_RZ, KP_RR, KP_BB};
     write_partials<T, 3, R>(acc, part, nb, sl);
+    if (S.ndone && blockIdx.x == 0 && threadIdx.x == 0) S.ndone[0] = 0;
     if (c.lead && threadIdx.x == 0) {
         ki[KI_DONE] = 0;
         ki[KI_ITERS] = 0;
@@ -482,7 +487,7 @@ __global__ void __launch_bounds__(KNT / R) k_kcg_spmv(int n, int it, int maxit, 
     kblock_sync();
     if (pv[3] != T(0)) return;
     const T rho = pv[0];
-    if (krylov_stop<T>(c.lead, it == 0, it, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
+    if (krylov_stop<T>(c.lead, it == 0, it, maxit, pv[1], pv[2], rtol, atol, false, ks, ki, S.ndone)) return;
     T beta = 0;
     if (it > 0) {
         T rho_old = ks[KS_RHO + ((it + 1) & 1)];
@@ -618,7 +623,7 @@ __global__ void __launch_bounds__(KNT / R) k_kbcg_a(int n, int it, int maxit, T 
         }
         kblock_sync();
         if (pv[3] != T(0)) return;
-        if (krylov_stop<T>(c.lead, true, 0, maxit, pv[1], pv[2], rtol, atol, false, ks, ki)) return;
+        if (krylov_stop<T>(c.lead, true, 0, maxit, pv[1], pv[2], rtol, atol, false, ks, ki, S.ndone)) return;
         if (c.lead && threadIdx.x == 0) ks[KS_RHO] = pv[0];
 #pragma unroll
         for (int q = 0; q < R; q++)
@@ -723,7 +728,7 @@ __global__ void __launch_bounds__(KNT / R) k_kbcg_b(int n, int it, int maxit, KR
     }
     kblock_sync();
     if (pv[2] != T(0)) return;
-    if (it > 0 && krylov_stop<T>(c.lead, false, it, maxit, pv[1], T(0), T(0), T(0), false, ks, ki)) return;
+    if (it > 0 && krylov_stop<T>(c.lead, false, it, maxit, pv[1], T(0), T(0), T(0), false, ks, ki, S.ndone)) return;
     const T rho = ks[KS_RHO + (it & 1)];
     const T alpha = pv[0] != T(0) ? rho / pv[0] : T(0);
     if (c.lead && threadIdx.x == 0) ks[KS_ALPHA + (it & 1)] = alpha;

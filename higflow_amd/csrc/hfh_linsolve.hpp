@@ -581,6 +581,11 @@ class DeviceSystem {
             // dispatch wait, plus the budget's launches past convergence)
             const char *eg = std::getenv("HFH_GRAPH");
             graph_on_ = eg && std::strcmp(eg, "1") == 0;
+            // the budget's launches past convergence cost more than the wait
+            // they save (sweep: 7.65 ms per mixed step with two waits per
+            // solve, 7.93 ms with one), so one wait is used with graphs only
+            const char *ef = std::getenv("HFH_FIRST_PASS");
+            first_pass_ = graph_on_ || (ef && std::strcmp(ef, "1") == 0);
             const char *er = std::getenv("HFH_RESID_DIAG");
             resid_diag_ = !(er && std::strcmp(er, "0") == 0);
         }
@@ -720,8 +725,8 @@ class DeviceSystem {
         for (int restart = 0; restart <= max_restarts; restart++) {
             if (restart > 0)  // keep the current x in case the restart does worse
                 HFH_CHECK(hipMemcpyAsync(xbak.p, x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-            if (restart == 0 && graph_on_) {
-                S.iters += run_graph64(spd, b, x, rtol, atol, maxit, hi, hs, rr, bb, st);
+            if (restart == 0 && first_pass_) {
+                S.iters += run_first_pass(spd, b, x, rtol, atol, maxit, hi, hs, rr, bb, st);
             } else {
                 k64.init<double>(spd, b, x, st);
                 S.iters += run_fused<double>(k64, spd, maxit - S.iters, rtol, atol, x, hi, hs, st);
@@ -766,26 +771,47 @@ class DeviceSystem {
         return S;
     }
 
-    // The first pass of an FP64 fused solve as one captured hipGraph: the
-    // initialisation, a budget of iterations (the last solve's count plus a
-    // quarter, rounded up to 8; launches after the stopping test return at
-    // once), the true-residual partials and the copies of the flags and
-    // partials to pinned memory.  The host waits once.  A solve that needs
-    // more iterations continues with host-checked chunks from where the
-    // graph stopped, so the iterates are those of the host-checked loop.
-    // One graph per (matrix form, vectors, budget, tolerances), kept until
-    // the matrix is set again.  HFH_GRAPH=1 turns it on; the default is the
-    // host-checked loop, which measured faster.
-    int run_graph64(bool spd, const double *b, double *x, double rtol, double atol, int maxit, int *hi, double *hs,
-                    std::vector<double> &rr, std::vector<double> &bb, hipStream_t st) {
+    // The first pass of an FP64 fused solve: the initialisation, a budget of
+    // iterations (the last solve's count plus a quarter; launches after the
+    // stopping test return at once), the true-residual partials and the
+    // copies of the flags and partials to pinned memory, then one host wait
+    // for both (the host-checked loop waits once for the flags, then once
+    // more for the residual).  A solve that needs more iterations continues
+    // with host-checked chunks from where the pass stopped, so the iterates
+    // are those of the host-checked loop.  HFH_GRAPH=1 captures the pass as
+    // a hipGraph (budget rounded up to 8; one graph per matrix form,
+    // vectors, budget and tolerances, kept until the matrix is set again);
+    // it measured slower than the launches from the host, so it is off by
+    // default; so is the single wait without a graph (HFH_FIRST_PASS=1).
+    int run_first_pass(bool spd, const double *b, double *x, double rtol, double atol, int maxit, int *hi,
+                       double *hs, std::vector<double> &rr, std::vector<double> &bb, hipStream_t st) {
         const int want = k64.pred + 2 + k64.pred / 4;
-        const int budget = std::min(maxit + 1, std::max(8, (want + 7) / 8 * 8));
+        double *prr = reinterpret_cast<double *>(hpin_ + off_rr_);
+        int budget;
+        if (!graph_on_) {
+            // the same launches from the host: one wait for the flags and
+            // the true residual instead of one for each
+            budget = std::min(maxit + 1, std::max(2, want));
+            k64.init<double>(spd, b, x, st);
+            k64.iterate(spd, 0, budget, maxit, rtol, atol, x, st);
+            true_residual(x, b, st);
+            copy_status<double>(k64, st);
+            HFH_CHECK(hipMemcpyAsync(prr, part1.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipMemcpyAsync(prr + nb, part2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            HFH_CHECK(hipStreamSynchronize(st));
+        } else {
+            budget = std::min(maxit + 1, std::max(8, (want + 7) / 8 * 8));
+            launch_graph64(spd, b, x, rtol, atol, maxit, budget, prr, st);
+        }
+        return finish_first_pass(spd, b, x, rtol, atol, maxit, budget, hi, hs, rr, bb, st);
+    }
+    void launch_graph64(bool spd, const double *b, double *x, double rtol, double atol, int maxit, int budget,
+                        double *prr, hipStream_t st) {
         hipGraphExec_t exec = nullptr;
         for (auto &g : graphs_)
             if (g.spd == spd && g.a32 == k64.use_amg32 && g.b == b && g.x == x && g.budget == budget &&
                 g.rtol == rtol && g.atol == atol && g.maxit == maxit)
                 exec = g.exec;
-        double *prr = reinterpret_cast<double *>(hpin_ + off_rr_);
         if (!exec) {
             if (graphs_.size() >= 16) clear_graphs();
             hipGraph_t graph = nullptr;
@@ -803,6 +829,10 @@ class DeviceSystem {
         }
         HFH_CHECK(hipGraphLaunch(exec, st));
         HFH_CHECK(hipStreamSynchronize(st));
+    }
+    int finish_first_pass(bool spd, const double *b, double *x, double rtol, double atol, int maxit, int budget,
+                          int *hi, double *hs, std::vector<double> &rr, std::vector<double> &bb, hipStream_t st) {
+        double *prr = reinterpret_cast<double *>(hpin_ + off_rr_);
         parse_status<double>(hi, hs);
         int it = budget;
         if (!hi[KI_DONE] && it <= maxit) {
@@ -1195,6 +1225,7 @@ class DeviceSystem {
     char *hpin_ = nullptr;  // pinned status: flags at 0, fused scalars at 64, outer scalars at 256
     int off_rr_ = 512;      // pinned true-residual partials (r.r then b.b, nb each)
     bool graph_on_ = false;  // first pass of FP64 fused solves as a captured graph (HFH_GRAPH=1)
+    bool first_pass_ = false;  // one wait for flags and true residual (HFH_FIRST_PASS=1, or with graphs)
     bool resid_diag_ = true;  // true residuals through the diagonal-form rows (HFH_RESID_DIAG)
     struct GraphEntry {
         bool spd, a32;
