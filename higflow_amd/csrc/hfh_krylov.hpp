@@ -350,6 +350,29 @@ __device__ inline void row_sums_r(const KRows<T> &A, const int (&i)[R], const bo
         }
         len = max(len, re[r] - rb[r]);
     }
+    if constexpr (R == 1) {
+        // one row: the loop bounds are the row's own
+#pragma unroll 1
+        for (int k0 = rb[0] + klane<L>(); k0 < re[0]; k0 += U * L) {
+            int j[U];
+            T a[U];
+            bool ok[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int k = k0 + q * L;
+                ok[q] = k < re[0];
+                j[q] = ok[q] ? base[0] + cp[0][k] : 0;
+                a[q] = ok[q] ? vp[0][k] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++)
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const T x = vec[m][j[q]];
+                    if (ok[q]) out[0][m] += a[q] * x;
+                }
+        }
+    } else {
 #pragma unroll 1
     for (int k0 = klane<L>(); k0 < len; k0 += U * L) {
         int j[R][U];
@@ -373,6 +396,7 @@ __device__ inline void row_sums_r(const KRows<T> &A, const int (&i)[R], const bo
                     const T x = vec[m][j[r][q]];
                     if (ok[r][q]) out[r][m] += a[r][q] * x;
                 }
+    }
     }
 #pragma unroll
     for (int r = 0; r < R; r++)
