@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "hfh_krylov.hpp"
+#include "hfh_progress.hpp"
 
 namespace hfh {
 
@@ -200,6 +201,7 @@ inline bool sa_build(const HCsr &A0, std::vector<AmgHostLevel> &lv, std::vector<
     HCsr A = A0;
     double nz = 0;
     while (A.nrow > coarse_max && (int)lv.size() < max_levels) {
+        progress("AMG hierarchy levels", (long)lv.size(), (long)max_levels);
         nz += (double)A.v.size();
         if (max_cx > 0 && nz > max_cx * (double)std::max<size_t>(A0.v.size(), 1)) return false;
         AmgHostLevel L;

@@ -5,6 +5,7 @@
 #include <array>
 #include <stdexcept>
 #include <atomic>
+#include <limits>
 #include <thread>
 
 #include "hfh_mesh.hpp"
@@ -42,10 +43,12 @@ struct CachedWls {
     bool use_dir = false;
 };
 
-// Per-thread WLS work space.
+// Per-thread WLS work space.  `items` holds the kept candidates (a heap
+// while a search runs), `count` the candidates accepted so far.
 struct Scratch {
-    std::vector<WItem> items, tmp;
+    std::vector<WItem> items;
     double max_dist = 0;
+    int count = 0;
     std::unique_ptr<Wls> wls, bcwls;
 };
 
@@ -280,30 +283,63 @@ class Domain {
         return false;
     }
 
-    static WItem *new_item(Scratch &sc, int maxpts, double dist) {  // domain.c:388-412
-        if (dist >= sc.max_dist) {
-            if ((int)sc.items.size() >= maxpts) return nullptr;
-            sc.max_dist = dist;
+    // Candidate lists (domain.c:373-412).  The reference appends a candidate
+    // unless the list already holds maxpts and the candidate is no nearer
+    // than the farthest one taken while the list filled, sorts the list with
+    // qsort on distance (glibc's merge sort: of two equal distances the later
+    // candidate comes first) and keeps the first maxpts.  Only those are ever
+    // read, so a heap of the maxpts first candidates under that same order
+    // (distance, then later first) is kept instead of the whole list: same
+    // points, same order.
+    static bool before(const WItem &a, const WItem &b) {
+        return a.dist < b.dist || (a.dist == b.dist && a.pos > b.pos);
+    }
+    static void begin_list(Scratch &sc) {
+        sc.items.clear();
+        sc.max_dist = 0.0;
+        sc.count = 0;
+    }
+    // the reference's append test, without its side effect
+    static bool rejected(const Scratch &sc, int maxpts, double dist) {
+        return dist >= sc.max_dist && sc.count >= maxpts;
+    }
+    static void accept(Scratch &sc, int maxpts, WItem &it) {
+        if (it.dist >= sc.max_dist) sc.max_dist = it.dist;
+        it.pos = sc.count++;
+        if ((int)sc.items.size() < maxpts) {
+            sc.items.push_back(it);
+            std::push_heap(sc.items.begin(), sc.items.end(), before);
+        } else if (it.dist <= sc.items.front().dist) {  // a later equal distance ranks first
+            std::pop_heap(sc.items.begin(), sc.items.end(), before);
+            sc.items.back() = it;
+            std::push_heap(sc.items.begin(), sc.items.end(), before);
         }
-        sc.items.emplace_back();
-        sc.items.back().dist = dist;
-        return &sc.items.back();
+    }
+    static void end_list(Scratch &sc) { std::sort(sc.items.begin(), sc.items.end(), before); }
+    // Once maxpts candidates are kept, one farther than the farthest kept
+    // cannot enter: squared bound for the tree walk, with a relative margin
+    // far above the rounding of the distances (boxes at the bound are walked).
+    static double far2(const Scratch &sc, int maxpts) {
+        if (sc.count < maxpts) return std::numeric_limits<double>::infinity();
+        const double r = sc.items.front().dist;
+        return r * r * (1.0 + 1e-9);
     }
 
     static void search_tree_cells(Scratch &sc, const Tree<D> &t, const IdMap &m, const Vec<D> &x,
                                   const Vec<D> &lo, const Vec<D> &hi, int maxpts, int bcidx, unsigned char type) {
-        t.bbox_cells(lo, hi, [&](int c) {
-            int id = m.lookup(t.n[c].cid);
-            if (id < 0) return;
+        t.bbox_cells_near(lo, hi, x, [&]() { return far2(sc, maxpts); }, [&](int c) {
             Vec<D> cc = t.center(c);
             double dist = distance<D>(x, cc);
-            WItem *it = new_item(sc, maxpts, dist);
-            if (it) {
-                it->id = id;
-                it->bc = bcidx;
-                it->type = type;
-                for (int d = 0; d < D; d++) it->x[d] = cc[d];
-            }
+            if (rejected(sc, maxpts, dist)) return;
+            int id = m.lookup(t.n[c].cid);
+            if (id < 0) return;
+            WItem it;
+            it.dist = dist;
+            it.id = id;
+            it.bc = bcidx;
+            it.type = type;
+            for (int d = 0; d < D; d++) it.x[d] = cc[d];
+            accept(sc, maxpts, it);
         });
     }
 
@@ -313,18 +349,19 @@ class Domain {
             return;
         }
         for (auto *t : trees) {
-            t->bbox_facets(interest, lo, hi, [&](const Facet<D> &f) {
-                int id = map.lookup(t->n[f.cell].fid[f.dim * 2 + f.dir]);
-                if (id < 0) return;
+            t->bbox_facets_near(interest, lo, hi, x, [&]() { return far2(sc, maxpts_); }, [&](const Facet<D> &f) {
                 Vec<D> fc = t->facet_center(f);
                 double dist = distance<D>(x, fc);
-                WItem *it = new_item(sc, maxpts_, dist);
-                if (it) {
-                    it->id = id;
-                    it->bc = -1;
-                    it->type = 0;
-                    for (int d = 0; d < D; d++) it->x[d] = fc[d];
-                }
+                if (rejected(sc, maxpts_, dist)) return;
+                int id = map.lookup(t->n[f.cell].fid[f.dim * 2 + f.dir]);
+                if (id < 0) return;
+                WItem it;
+                it.dist = dist;
+                it.id = id;
+                it.bc = -1;
+                it.type = 0;
+                for (int d = 0; d < D; d++) it.x[d] = fc[d];
+                accept(sc, maxpts_, it);
             });
         }
     }
@@ -376,28 +413,20 @@ class Domain {
             return out;
         }
         Scratch &sc = sc_;
-        sc.items.clear();
-        sc.max_dist = 0.0;
+        begin_list(sc);
         Vec<D> lo, hi;
         for (int i = 0; i < D; i++) {
             lo[i] = boxc[i] - 3.0 * bd;
             hi[i] = boxc[i] + 3.0 * bd;
         }
         search_tree_cells(sc, *b->tree, b->map, px, lo, hi, bc_maxpts_, -1, 2);
-        sort_items(sc);
-        if ((int)sc.items.size() > bc_maxpts_) sc.items.resize(bc_maxpts_);
+        end_list(sc);
         for (auto &it : sc.items)
             for (int d = b->pdir + 1; d < D; d++) it.x[d - 1] = it.x[d];
         std::vector<double> w(sc.items.size() + 1);
         bcwls(sc).weights((int)sc.items.size(), sc.items.data(), px.v, w.data());
         for (size_t i = 0; i < sc.items.size(); i++) out.emplace_back(b->base + sc.items[i].id, w[i]);
         return out;
-    }
-
-    static void sort_items(Scratch &sc) {
-        if (sc.items.size() <= 1) return;
-        sc.tmp.resize(sc.items.size());
-        merge_sort_items(sc.items.data(), (int)sc.items.size(), sc.tmp.data());
     }
 
     struct Proj {
@@ -550,8 +579,7 @@ class Domain {
         Vec<D> x;
         for (int k = 0; k < D; k++) x[k] = cw.x[k];
         const double delta = cw.delta;
-        sc.items.clear();
-        sc.max_dist = 0.0;
+        begin_list(sc);
         Vec<D> lo, hi;
         for (int k = 0; k < D; k++) {
             lo[k] = x[k] - 10.0 * delta;
@@ -561,8 +589,7 @@ class Domain {
         if (cw.use_dir)
             for (size_t i = 0; i < dir.size(); i++)
                 search_tree_cells(sc, *dir[i]->tree, dir[i]->map, x, lo, hi, maxpts_, (int)i, 1);
-        sort_items(sc);
-        if ((int)sc.items.size() > maxpts_) sc.items.resize(maxpts_);
+        end_list(sc);
         std::vector<double> w(sc.items.size() + 1);
         wls(sc).weights((int)sc.items.size(), sc.items.data(), x.v, w.data());
         cw.ids.clear();
@@ -669,7 +696,10 @@ class Domain {
         std::atomic<size_t> next{0};
         auto run = [&]() {
             Scratch sc;
-            for (size_t i; (i = next.fetch_add(1)) < work.size();) compute_wls(*work[i], sc);
+            for (size_t i; (i = next.fetch_add(1)) < work.size();) {
+                compute_wls(*work[i], sc);
+                if ((i & 4095) == 0) progress("least-squares fits", (long)i, (long)work.size());
+            }
         };
         int nt = (int)std::min<size_t>((size_t)std::max(1, nthreads), work.size());
         std::vector<std::thread> th;

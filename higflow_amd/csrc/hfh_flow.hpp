@@ -429,6 +429,16 @@ class Flow : public FlowBase {
 template <int D>
 int Flow<D>::setup(const Params &p) {
     if (ready_ || trees_.empty()) return -1;
+    // HFH_VERBOSE=1: seconds spent in each set-up phase
+    const char *evs = std::getenv("HFH_VERBOSE");
+    const bool vset = evs && std::strcmp(evs, "1") == 0;
+    auto t_last = std::chrono::steady_clock::now();
+    progress_start();
+    auto phase = [&](const char *name) {
+        auto now = std::chrono::steady_clock::now();
+        if (vset) std::fprintf(stderr, "hfh: set-up %s %.3f s\n", name, std::chrono::duration<double>(now - t_last).count());
+        t_last = now;
+    };
     par_ = p;
     if (par_.order <= 0) par_.order = 2;
     if (par_.maxit <= 0) par_.maxit = 20000;
@@ -500,6 +510,7 @@ int Flow<D>::setup(const Params &p) {
             bcu_[d].push_back(std::move(b));
         }
     bcval_.assign(base + 1, 0.0);
+    phase("maps and boundaries");
     Nc_ = sdp_->size();
     for (int d = 0; d < D; d++) Nf_[d] = sfdu_[d]->size();
     {
@@ -550,11 +561,15 @@ int Flow<D>::setup(const Params &p) {
         }
         for (auto *dm : doms) dm->begin_deferred();
         record();
+        phase("stencil requests");
         for (auto *dm : doms) dm->resolve_pending(nth);
+        phase("least-squares solves");
         reset_records();
     }
     record();
+    phase("row recording");
     finalize_layout();
+    phase("layout");
     return 0;
 }
 
@@ -604,6 +619,7 @@ void Flow<D>::record_momentum(bool ve) {
         momFlags_[dim].assign(nf, 0);
         momDelta_[dim].assign((size_t)D * nf, 0.0);
         for (int f = 0; f < nf; f++) {
+            if ((f & 1023) == 0) progress("momentum rows", f, nf);
             Vec<D> fc = fu.center_of(f), fd = fu.delta_of(f);
             for (int k = 0; k < D; k++) momDelta_[dim][(size_t)k * nf + f] = fd[k];
             auto S = [&](int k) -> int & { return momSlots_[dim][(size_t)k * nf + f]; };
@@ -761,6 +777,7 @@ void Flow<D>::record_pressure() {
     std::vector<std::vector<std::pair<int, double>>> rows(n);
     std::vector<Combo> bcombo(n);
     for (int c = 0; c < n; c++) {
+        if ((c & 1023) == 0) progress("pressure rows", c, n);
         Vec<D> cc = sp.center_of(c), cd = sp.delta_of(c);
         Combo &cb = bcombo[c];
         for (int dim = 0; dim < D; dim++) {
@@ -858,6 +875,7 @@ void Flow<D>::record_projection() {
     for (int dim = 0; dim < D; dim++) {
         Domain<D> &fu = *sfdu_[dim];
         for (int f = 0; f < Nf_[dim]; f++) {
+            if ((f & 1023) == 0) progress("projection rows", f, Nf_[dim]);
             Vec<D> fc = fu.center_of(f), fd = fu.delta_of(f);
             Vec<D> pl = fc, pr = fc;
             pl[dim] = fc[dim] - 0.5 * fd[dim];

@@ -12,6 +12,7 @@ void Flow<D>::record_ve() {
     int ne = se.size();
     // velocity-derivative tensor, one combined row per (cell, i, j)
     for (int c = 0; c < ne; c++) {
+        if ((c & 1023) == 0) progress("tensor rows", c, ne);
         Vec<D> cc = se.center_of(c), cd = se.delta_of(c);
         for (int dim = 0; dim < D; dim++)
             for (int dim2 = 0; dim2 < D; dim2++) {

@@ -23,6 +23,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include "hfh_progress.hpp"
+
 namespace hfh {
 
 constexpr double EPSDELTA = 1.0e-8;  // higtree/src/utils.h:16
@@ -34,6 +36,7 @@ inline bool flt_eq(double x, double y) { return std::fabs(x - y) < EPSMACH; }
 inline bool pos_le(double x, double y) { return x < y + EPSDELTA; }
 inline bool pos_ge(double x, double y) { return x > y - EPSDELTA; }
 inline double rmax(double a, double b) { return a > b ? a : b; }
+
 
 enum BcType { DIRICHLET = 0, NEUMANN = 1 };
 
@@ -311,7 +314,70 @@ class Tree {
         });
     }
 
+    // bbox_cells / bbox_facets in the same order, skipping every child box
+    // whose squared distance from x exceeds far2() (read before each child,
+    // never increasing).  Children of a node form a regular grid, so a box
+    // too far in the leading dimensions ends its run along the last one, as
+    // does a box too far beyond x in the last dimension.
+    template <class F, class G>
+    void bbox_cells_near(const Vec<D> &lo, const Vec<D> &hi, const Vec<D> &x, G &&far2, F &&fn) const {
+        if (intersects(0, lo, hi)) near_rec(0, lo, hi, x, far2, fn);
+    }
+
+    template <class F, class G>
+    void bbox_facets_near(const int *interest, const Vec<D> &lo, const Vec<D> &hi, const Vec<D> &x, G &&far2,
+                          F &&fn) const {
+        bbox_cells_near(lo, hi, x, far2, [&](int c) {
+            for (int dim = 0; dim < D; dim++)
+                for (int dir = 0; dir < 2; dir++) {
+                    if (!interest[dim] || n[c].fid[dim * 2 + dir] <= 0) continue;
+                    Facet<D> f{c, dim, dir};
+                    if (facet_intersects(f, lo, hi)) fn(f);
+                }
+        });
+    }
+
   private:
+    template <class F, class G>
+    void near_rec(int c, const Vec<D> &lo, const Vec<D> &hi, const Vec<D> &x, G &far2, F &fn) const {
+        if (n[c].child0 < 0) {
+            fn(c);
+            return;
+        }
+        int pl[D], ph[D], ps[D];
+        coords_of_point(c, lo, pl);
+        coords_of_point(c, hi, ph);
+        for (int i = 0; i < D; i++) {
+            if (pl[i] < 0) pl[i] = 0;
+            if (ph[i] >= n[c].nc[i]) ph[i] = n[c].nc[i] - 1;
+            ps[i] = pl[i];
+        }
+        for (;;) {
+            const int ch = n[c].child0 + toposition<D>(n[c].nc, ps);
+            const Node<D> &q = n[ch];
+            const double lim = far2();
+            double a = 0;
+            for (int i = 0; i < D - 1; i++) {
+                const double g = rmax(rmax(q.lo[i] - x[i], x[i] - q.hi[i]), 0.0);
+                a += g * g;
+            }
+            const double g = rmax(rmax(q.lo[D - 1] - x[D - 1], x[D - 1] - q.hi[D - 1]), 0.0);
+            if (a > lim) ps[D - 1] = ph[D - 1];
+            else if (a + g * g > lim) {
+                if (q.lo[D - 1] > x[D - 1]) ps[D - 1] = ph[D - 1];
+            } else {
+                near_rec(ch, lo, hi, x, far2, fn);
+            }
+            int i = D - 1;
+            while (i >= 0 && ps[i] >= ph[i]) {
+                ps[i] = pl[i];
+                i--;
+            }
+            if (i < 0) break;
+            ps[i]++;
+        }
+    }
+
     template <class F>
     void bbox_rec(int c, const Vec<D> &lo, const Vec<D> &hi, F &fn) const {
         if (n[c].child0 < 0) {
@@ -466,6 +532,51 @@ struct Boundary {
     }
 };
 
+// Index over an append-only list of ids that accumulates repeated ids
+// (stencils, row combinations): up to 16 entries the list is scanned, past
+// that an open-addressed table holds position + 1 per slot.  Positions are
+// those of the list, so first-insertion order and the order of the sums
+// are those of a plain scan.
+class IdIndex {
+  public:
+    void clear() { slot_.clear(); }
+    // position of id among the n entries (id of entry i: key(i)), or -1
+    template <class K>
+    int find(int id, size_t n, const K &key) const {
+        if (slot_.empty()) {
+            for (size_t i = 0; i < n; i++)
+                if (key(i) == id) return (int)i;
+            return -1;
+        }
+        const size_t m = slot_.size() - 1;
+        for (size_t h = hash(id) & m; slot_[h]; h = (h + 1) & m)
+            if (key((size_t)slot_[h] - 1) == id) return slot_[h] - 1;
+        return -1;
+    }
+    // id was appended as entry n - 1
+    template <class K>
+    void appended(int id, size_t n, const K &key) {
+        if (slot_.empty() ? n > 16 : 2 * n > slot_.size()) {
+            size_t cap = 64;
+            while (cap < 4 * n) cap *= 2;
+            slot_.assign(cap, 0);
+            for (size_t i = 0; i < n; i++) put(key(i), (int)i + 1);
+        } else if (!slot_.empty()) {
+            put(id, (int)n);
+        }
+    }
+
+  private:
+    std::vector<int> slot_;
+    static size_t hash(int id) { return (size_t)((uint32_t)id * 0x9E3779B1u) >> 7; }
+    void put(int id, int v) {
+        const size_t m = slot_.size() - 1;
+        size_t h = hash(id) & m;
+        while (slot_[h]) h = (h + 1) & m;
+        slot_[h] = v;
+    }
+};
+
 // Affine point stencil with boundary values kept symbolic:
 //   value = -rhs0 - sum_b bc[b].second * bcval[bc[b].first] + sum_k w_k f[id_k]
 struct Stencil {
@@ -478,15 +589,18 @@ struct Stencil {
         w.clear();
         bc.clear();
         rhs0 = 0.0;
+        index_.clear();
     }
     void add(int id, double v) {  // domain.c:1994-2010
-        for (size_t i = 0; i < ids.size(); i++)
-            if (ids[i] == id) {
-                w[i] += v;
-                return;
-            }
+        auto key = [this](size_t i) { return ids[i]; };
+        const int i = index_.find(id, ids.size(), key);
+        if (i >= 0) {
+            w[i] += v;
+            return;
+        }
         ids.push_back(id);
         w.push_back(v);
+        index_.appended(id, ids.size(), key);
     }
     void add_rhs_bc(int slot, double coef) { bc.emplace_back(slot, coef); }
     double rhs(const std::vector<double> &bcval) const {
@@ -494,6 +608,9 @@ struct Stencil {
         for (auto &b : bc) r += b.second * bcval[b.first];
         return r;
     }
+
+  private:
+    IdIndex index_;
 };
 
 struct WItem {
@@ -501,6 +618,7 @@ struct WItem {
     double dist;
     int bc;  // boundary index in the domain's Dirichlet list, -1 otherwise
     int id;
+    int pos;  // place in the order candidates were accepted
     unsigned char type;
 };
 
@@ -621,28 +739,5 @@ class Wls {
             for (j = 0, x[i] = 0.0; j < m; j++) x[i] += Q(j, i) * y[j];
     }
 };
-
-// glibc 2.35 qsort (msort) order for the reference comparator
-// `a.dist < b.dist ? -1 : 1` (domain.c:373-377): stable-merge with ties
-// taken from the right run first.
-inline void merge_sort_items(WItem *b, int n, WItem *tmp) {
-    if (n <= 1) return;
-    int n1 = n / 2, n2 = n - n1;
-    WItem *b1 = b, *b2 = b + n1;
-    merge_sort_items(b1, n1, tmp);
-    merge_sort_items(b2, n2, tmp);
-    WItem *t = tmp;
-    while (n1 > 0 && n2 > 0) {
-        if (b1->dist < b2->dist) {
-            *t++ = *b1++;
-            n1--;
-        } else {
-            *t++ = *b2++;
-            n2--;
-        }
-    }
-    if (n1 > 0) std::memcpy(t, b1, (size_t)n1 * sizeof *t);
-    std::memcpy(b, tmp, (size_t)(n - n2) * sizeof *b);
-}
 
 }  // namespace hfh

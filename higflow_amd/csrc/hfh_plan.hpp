@@ -85,33 +85,28 @@ struct Combo {
     std::vector<std::pair<int, double>> bc;
     double rhs0 = 0.0;
     void add(const Stencil &s, int field, double coef) {
-        for (size_t k = 0; k < s.ids.size(); k++) {
-            int c = field + s.ids[k];
-            bool found = false;
-            for (auto &e : cols)
-                if (e.first == c) {
-                    e.second += coef * s.w[k];
-                    found = true;
-                    break;
-                }
-            if (!found) cols.emplace_back(c, coef * s.w[k]);
-        }
+        for (size_t k = 0; k < s.ids.size(); k++) add_state(field + s.ids[k], coef * s.w[k]);
         for (auto &b : s.bc) bc.emplace_back(b.first, coef * b.second);
         rhs0 += coef * s.rhs0;
     }
     void add_state(int off, double coef) {
-        for (auto &e : cols)
-            if (e.first == off) {
-                e.second += coef;
-                return;
-            }
+        auto key = [this](size_t i) { return cols[i].first; };
+        const int i = index_.find(off, cols.size(), key);
+        if (i >= 0) {
+            cols[i].second += coef;
+            return;
+        }
         cols.emplace_back(off, coef);
+        index_.appended(off, cols.size(), key);
     }
     void scale(double s) {
         for (auto &e : cols) e.second *= s;
         for (auto &b : bc) b.second *= s;
         rhs0 *= s;
     }
+
+  private:
+    IdIndex index_;
 };
 
 }  // namespace hfh
