@@ -56,6 +56,13 @@ WORKLOADS = {
     # PCG32 table (sides 32..256, Re log-uniform in [1, 100], Wi in
     # [0.1, 2]; 1,225,216 cells), beta=0.5, dt=5e-4, semi-implicit Euler,
     # all stepped together (one launch per stage)
+    # 3D Newtonian lid-driven cavity on [0,1]^3: base n^3 cells and 2
+    # factor-2 levels in a 0.05-thick layer along every wall (each level's
+    # layer snapped outward to its parent grid), Re=400, lid u=1 on z=1,
+    # dt=2e-4, explicit Euler, 100 steps; 16^3 base: 89,272 leaves, 24^3
+    # base: 310,120 leaves
+    "cavity3d_16_wall2": dict(kind="wall", n=16, Re=400.0, dt=2e-4, steps=100),
+    "cavity3d_24_wall2": dict(kind="wall", n=24, Re=400.0, dt=2e-4, steps=100),
     "sweep64_ve_cavity": dict(kind="sweep", count=64, seed=0, dt=5e-4, steps=100,
                               roof=(13, "k_amg_prolong_post<float> (velocity V-cycle, fine level, mixed form)",
                                     "mixed")),
@@ -73,6 +80,8 @@ def make_case(spec, **tol):
         return cases.sweep(spec["count"], spec["seed"], dt=spec["dt"], **tol)
     if spec["kind"] == "cavity":
         return cases.cavity(spec["n"], Re=spec["Re"], dt=spec["dt"], **tol)
+    if spec["kind"] == "wall":
+        return cases.wall_layer_cavity(spec["n"], Re=spec["Re"], dt=spec["dt"], **tol)
     if spec["kind"] == "contraction":
         return cases.contraction(levels=spec["levels"], dt=spec["dt"], graded=spec.get("graded", False),
                                  h0=spec.get("h0", 0.1), **tol)
@@ -192,6 +201,8 @@ def main():
         print("bench.py: this build runs the projection step on one GPU; multi-GPU runs are not part of it "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    # set-up progress lines on stderr (3D meshes take minutes of host set-up)
+    os.environ.setdefault("HFH_PROGRESS", "1")
     spec = WORKLOADS[args.workload]
     if args.steps is None:
         args.steps = spec["steps"]
@@ -250,6 +261,7 @@ def main():
     for which, name, sim in ((9, "k_kbcg_a<double>", sim64), (11, "k_kcg_spmv<double>(velocity)", sim64),
                              (12, "k_kcg_spmv<float>(velocity)", simmx),
                              (13, "k_amg_prolong_post<float>(velocity, fine level)", simmx),
+                             (14, "k_amg_prolong_post<float>(pressure, fine level)", simmx),
                              (10, "k_kbcg_b<double>", sim64), (6, "k_kbcg_b<double>(pressure)", sim64),
                              (7, "k_kbcg_b<float>(pressure)", simmx), (0, "k_spmv_dot<double>", sim64),
                              (1, "k_momentum", sim64), (2, "k_interp(momentum rows)", sim64),

@@ -973,6 +973,17 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
             *bytes = by;
             return 0;
         }
+        case 14: {
+            // the same launch of the pressure system's FP32 V-cycle
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (!psys_.k32.amg_on()) return -1;
+            const float ms = psys_.k32.amg.probe_pp(reps, &by, st_);
+            if (ms < 0) return -1;
+            *avg_ms = ms;
+            *bytes = by;
+            return 0;
+        }
         case 8: {
             // plain copy of 8 Mi doubles (64 MiB in, 64 MiB out): the
             // reference point that scales the hardware byte counters
