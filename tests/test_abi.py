@@ -43,6 +43,9 @@ def host_only(monkeypatch):
 
 @pytest.mark.parametrize("make", [lambda: cases.cavity(16), lambda: cases.newt_channel(nx=40, ny=10),
                                   lambda: cases.cavity(4, dim=3),
+                                  # 8^3: more candidates than maxpts with many equal distances, so
+                                  # the candidate heap fills and the bounded tree walk prunes
+                                  lambda: cases.cavity(8, dim=3),
                                   lambda: cases.wall_layer_cavity(4, dim=3, levels=1),
                                   lambda: cases.contraction(levels=2, h0=0.5, length=8.0),
                                   lambda: cases.newt_channel(nx=40, ny=10, tempdiscrtype=3)])
