@@ -299,9 +299,12 @@ inline int amg_lanes(long nnz, long rows) {
     // short rows take fewer lanes on large levels; a level of fewer rows is
     // bound by its launch latency, and more threads per row hide it better
     // (47,800-row contraction: the pressure solve 1.82 ms with 1-2 lanes
-    // against 1.69 with 4)
-    if (rows < AMG_LANES_MIN_ROWS) return 4;
+    // against 1.69 with 4).  Long rows (3D least-squares stencils, the
+    // Galerkin products of 3D aggregation) take 8 or 16 lanes.
     const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
+    if (avg > 48.0) return 16;
+    if (avg > 24.0) return 8;
+    if (rows < AMG_LANES_MIN_ROWS) return 4;
     return avg <= 3.0 ? 1 : avg <= 6.0 ? 2 : 4;
 }
 
@@ -566,14 +569,19 @@ class AmgPrecond {
             d.lq = amg_lanes((long)std::max(hl[l].P.v.size(), hl[l].AP.v.size()), d.n);
             if (const char *e = std::getenv("HFH_AMG_LANES")) {
                 const int f = std::atoi(e);
-                if (f == 1 || f == 2 || f == 4) d.la = d.lr = d.lp = d.lq = f;
+                if (f == 1 || f == 2 || f == 4 || f == 8 || f == 16) d.la = d.lr = d.lp = d.lq = f;
             }
+            if (const char *ev = std::getenv("HFH_VERBOSE"))
+                if (std::strcmp(ev, "1") == 0)
+                    std::fprintf(stderr, "hfh: AMG level %d: %d rows, %.1f entries per row; lanes A %d R %d P %d AP %d\n",
+                                 l, d.n, d.n ? (double)hl[l].A.v.size() / d.n : 0.0, d.la, d.lr, d.lp, d.lq);
             d.r.alloc(d.n);
             d.x.alloc(d.n);
             d.bnext.alloc(d.nc);
             d.xnext.alloc(d.nc);
         }
         upt(cinv_, cinv, st);
+        coarse_r_nnz_ = hl.empty() ? 0 : (long)hl.back().R.v.size();
         {
             std::vector<float> c32(cinv.begin(), cinv.end());
             cinv32_.alloc(c32.size());
@@ -651,6 +659,7 @@ class AmgPrecond {
     std::vector<Dev> lv_;
     KBuf<T> cinv_;
     KBuf<float> cinv32_;  // the dense coarsest inverse as read by k_amg_dense
+    long coarse_r_nnz_ = 0;  // entries of the last restriction
     KBuf<T> probe_y_;
 
     static void upi(KBuf<int> &d, const std::vector<int> &h, hipStream_t st) {
@@ -682,9 +691,16 @@ class AmgPrecond {
         if (l + 1 < nlev) {
             restrict_(d, ki, st);
             cycle<T, T>(l + 1, d.bnext.p, d.xnext.p, ki, st);
-        } else if (ncoarse <= 256) {
+        } else if (ncoarse <= 256 && coarse_r_nnz_ <= 32768) {
             k_amg_coarse<T><<<1, 1024, ncoarse * sizeof(T), st>>>(ncoarse, d.Rrp.p, d.Rci.p, d.Rv.p, d.r.p, cinv_.p,
                                                                    d.xnext.p, ki);
+        } else if (ncoarse <= 256) {
+            // a small coarsest level under a long last restriction (3D
+            // aggregation): restrict on many workgroups, then the same dense
+            // solve as above, a wave per row
+            restrict_(d, ki, st);
+            k_amg_dense<T, T><<<std::max(1, (ncoarse * 64 + ANT - 1) / ANT), ANT, 0, st>>>(
+                ncoarse, cinv_.p, d.bnext.p, d.xnext.p, ki);
         } else {
             // larger coarsest level: restrict, then a wave per dense row
             restrict_(d, ki, st);
@@ -726,8 +742,12 @@ class AmgPrecond {
             f(std::integral_constant<int, 1>{});
         else if (al == 2)
             f(std::integral_constant<int, 2>{});
-        else
+        else if (al == 4)
             f(std::integral_constant<int, 4>{});
+        else if (al == 8)
+            f(std::integral_constant<int, 8>{});
+        else
+            f(std::integral_constant<int, 16>{});
     }
 };
 

@@ -95,6 +95,7 @@ class FusedKrylov {
         Ah = Av;
         Ah.val = vhat.p;
         build_templates(hrp, hci, tv, th, st);
+        lane_rule_ = lanes_by_rule(hrp);
         for (KBuf<T> *b : {&r, &rh, &z, &p, &p2, &q, &v, &v2, &s, &t}) b->alloc(n);
         ks.alloc((size_t)KS_N * nseg);
         ki.alloc((size_t)KI_N * nseg);
@@ -617,13 +618,29 @@ class FusedKrylov {
         k_fill<T><<<nblocks((long)KS_N * nseg), BLOCK, 0, st>>>(KS_N * nseg, ks.p, T(0));
         return ta + tb;
     }
-    // picks the lanes per row L and rows per lane group R with the fastest
-    // iteration (launches A and B of BiCGStab, with the system's segments).
-    // The row sums and partial sums depend on L, not on R.  The choice is
-    // kept per matrix shape (rows, entries, segments, precision) for the
-    // life of the process, so solvers built later for the same system (a
-    // second run of the same case) sum in the same order and give the same
-    // iterates.  HFH_KRYLOV_LANES / HFH_KRYLOV_ROWS override the choice.
+    // Lanes per row by rule: the 99th percentile of the row lengths over 12
+    // entries per lane, as a power of two up to 8.  A wave waits for its
+    // longest row, so the long tail (rows of an AMR interface or of a 3D
+    // least-squares stencil) sets the lanes, not the mean.
+    int lane_rule_ = 1;
+    static int lanes_by_rule(const std::vector<int> &hrp) {
+        const int n = (int)hrp.size() - 1;
+        if (n <= 0) return 1;
+        std::vector<int> len(n);
+        for (int i = 0; i < n; i++) len[i] = hrp[i + 1] - hrp[i];
+        const size_t k = std::min<size_t>((size_t)n - 1, (size_t)(0.99 * n));
+        std::nth_element(len.begin(), len.begin() + k, len.end());
+        const int p99 = len[k];
+        return p99 <= 12 ? 1 : p99 <= 24 ? 2 : p99 <= 48 ? 4 : 8;
+    }
+    // Picks the lanes per row L and rows per lane group R.  The row sums
+    // and partial sums depend on L, not on R, so L comes from the rule above
+    // (the same on every run and machine) and only R is timed (launches A
+    // and B of BiCGStab, with the system's segments; R > 1 only for L <= 2).
+    // HFH_KRYLOV_TIMED_LANES=1 times L as well (the fastest shape of this
+    // machine; results then depend on the timing).  The choice is kept per
+    // matrix shape (rows, entries, segments, precision) for the life of the
+    // process.  HFH_KRYLOV_LANES / HFH_KRYLOV_ROWS override the choice.
     void tune(hipStream_t st) {
         static std::mutex mu;
         static std::map<std::tuple<int, long, int, int>, int> chosen;
@@ -632,8 +649,15 @@ class FusedKrylov {
             int last = 0;
             if (hipMemcpy(&last, rp + n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) nz = last;
         }
-        const auto key = std::make_tuple(n, nz, nseg, (int)sizeof(T));
-        int best = 11;
+        const char *tl = std::getenv("HFH_KRYLOV_TIMED_LANES");
+        const bool timed_lanes = tl && std::strcmp(tl, "1") == 0;
+        int lrule = lane_rule_;
+        if (const char *e = std::getenv("HFH_KRYLOV_LANES")) {
+            int f = std::atoi(e);
+            if (f == 1 || f == 2 || f == 4 || f == 8) lrule = f;
+        }
+        const auto key = std::make_tuple(n, nz, nseg, (int)sizeof(T) + (timed_lanes ? 0 : 16 * lrule));
+        int best = 10 * lrule + 1;
         bool known = false;
         {
             std::lock_guard<std::mutex> g(mu);
@@ -647,6 +671,8 @@ class FusedKrylov {
             const int shapes[8] = {11, 12, 14, 21, 22, 24, 41, 81};
             float tb = 1e30f;
             for (int sh : shapes) {
+                if (!timed_lanes && sh / 10 != lrule) continue;
+                if (!timed_lanes && lrule > 2) break;  // one shape: nothing to time
                 lanes = sh / 10;
                 rows = sh % 10;
                 float tm = 0;

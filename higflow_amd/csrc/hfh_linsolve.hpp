@@ -912,7 +912,12 @@ class DeviceSystem {
                 S = F;
             }
         } else {
-            k64.use_amg32 = mixed_amg32_;
+            // the FP32 V-cycle only under PCG.  Under BiCGStab its rounding
+            // (a preconditioner that is not exactly linear) made the 3D
+            // wall-layer pressure converge erratically (111 iterations
+            // against 36 with the FP64 cycle) and stop on its updated
+            // residual with a wrong x.
+            k64.use_amg32 = mixed_amg32_ && spd;
             S = fused64(spd, b, x, rtol, atol, maxit, st);
             k64.use_amg32 = false;
             S.fallback = 2;
