@@ -62,8 +62,10 @@ WORKLOADS = {
     # dt=2e-4, explicit Euler, 100 steps, solves to 1e-9 (the stated
     # refinement target of this configuration); 16^3 base: 89,272 leaves,
     # 24^3 base: 310,120 leaves
-    "cavity3d_16_wall2": dict(kind="wall", n=16, Re=400.0, dt=2e-4, steps=100, tol=1e-9),
-    "cavity3d_24_wall2": dict(kind="wall", n=24, Re=400.0, dt=2e-4, steps=100, tol=1e-9),
+    "cavity3d_16_wall2": dict(kind="wall", n=16, Re=400.0, dt=2e-4, steps=100, tol=1e-9,
+                              roof=(15, "k_amg_resid<double> (pressure V-cycle, fine level, FP64 form)", "fp64")),
+    "cavity3d_24_wall2": dict(kind="wall", n=24, Re=400.0, dt=2e-4, steps=100, tol=1e-9,
+                              roof=(15, "k_amg_resid<double> (pressure V-cycle, fine level, FP64 form)", "fp64")),
     "sweep64_ve_cavity": dict(kind="sweep", count=64, seed=0, dt=5e-4, steps=100,
                               roof=(13, "k_amg_prolong_post<float> (velocity V-cycle, fine level, mixed form)",
                                     "mixed")),
@@ -264,6 +266,7 @@ def main():
                              (12, "k_kcg_spmv<float>(velocity)", simmx),
                              (13, "k_amg_prolong_post<float>(velocity, fine level)", simmx),
                              (14, "k_amg_prolong_post<float>(pressure, fine level)", simmx),
+                             (15, "k_amg_resid<double>(pressure, fine level)", sim64),
                              (10, "k_kbcg_b<double>", sim64), (6, "k_kbcg_b<double>(pressure)", sim64),
                              (7, "k_kbcg_b<float>(pressure)", simmx), (0, "k_spmv_dot<double>", sim64),
                              (1, "k_momentum", sim64), (2, "k_interp(momentum rows)", sim64),

@@ -638,6 +638,34 @@ class AmgPrecond {
         return reps > 0 ? ms / reps : 0.0f;
     }
     int fine_lanes_pp() const { return lv_.empty() ? 0 : lv_[0].lq; }
+    // the same for the fine level's first launch, r = b - A (omega D^-1 b)
+    // on CSR rows (-1 when the fine level runs in the diagonal form)
+    float probe_resid(int reps, long *bytes, hipStream_t st) {
+        if (!ready || nlev == 0 || A0.dw) return -1.0f;
+        Dev &d = lv_[0];
+        probe_y_.alloc(d.n);
+        k_fill<T><<<nblocks(d.n), BLOCK, 0, st>>>(d.n, probe_y_.p, T(1));
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            throw std::runtime_error("hipEventCreate failed");
+        (void)hipEventRecord(a, st);
+        with_lanes(d.la, [&](auto C) {
+            constexpr int AL = decltype(C)::value;
+            for (int q = 0; q < reps; q++)
+                k_amg_resid<T, T, AL><<<grid(d.n, AL), ANT, 0, st>>>(d.n, d.Arp.p, d.Aci.p, d.Av.p, d.dinv.p, omega,
+                                                                     probe_y_.p, d.r.p, nullptr);
+        });
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        // A's rows as stored with their row pointers; b and D^-1 read and r
+        // written once per row
+        *bytes = (long)d.Av.n * (4 + (long)sizeof(T)) + (long)(d.n + 1) * 4 + 3L * d.n * (long)sizeof(T);
+        return reps > 0 ? ms / reps : 0.0f;
+    }
 
     long bytes_per_cycle() const {
         long by = 0;

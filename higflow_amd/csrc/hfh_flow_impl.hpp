@@ -973,12 +973,14 @@ int Flow<D>::probe(int which, int reps, double *avg_ms, long *bytes) {
             *bytes = by;
             return 0;
         }
-        case 14: {
-            // the same launch of the pressure system's FP32 V-cycle
+        case 14:
+        case 15: {
+            // the same launch of the pressure system's FP32 V-cycle (14);
+            // the fine-level residual launch of its FP64 V-cycle (15)
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
-            if (!psys_.k32.amg_on()) return -1;
-            const float ms = psys_.k32.amg.probe_pp(reps, &by, st_);
+            const float ms = which == 14 ? (psys_.k32.amg_on() ? psys_.k32.amg.probe_pp(reps, &by, st_) : -1.0f)
+                                         : (psys_.k64.amg_on() ? psys_.k64.amg.probe_resid(reps, &by, st_) : -1.0f);
             if (ms < 0) return -1;
             *avg_ms = ms;
             *bytes = by;

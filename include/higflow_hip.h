@@ -218,7 +218,8 @@ int hfh_set_timing(hfh_solver *s, int on);
  * the step has none), 11 / 12 the fused PCG SpMV launch of that velocity
  * system in FP64 / FP32, 13 the fine-level prolongation + post-smoothing
  * launch of that system's FP32 V-cycle (AMG-PCG velocity systems), 14 the
- * same launch of the pressure system's FP32 V-cycle.
+ * same launch of the pressure system's FP32 V-cycle, 15 the fine-level
+ * residual launch of the pressure system's FP64 V-cycle (CSR rows).
  * Average ms per launch and the algorithmic bytes of one launch; -1 when
  * the step has no such launch.  Call between steps. */
 int hfh_probe(hfh_solver *s, int which, int reps, double *avg_ms, long *bytes);
