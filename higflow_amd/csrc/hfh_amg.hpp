@@ -300,8 +300,11 @@ inline int amg_lanes(long nnz, long rows) {
     // bound by its launch latency, and more threads per row hide it better
     // (47,800-row contraction: the pressure solve 1.82 ms with 1-2 lanes
     // against 1.69 with 4).  Long rows (3D least-squares stencils, the
-    // Galerkin products of 3D aggregation) take 8 or 16 lanes.
+    // Galerkin products of 3D aggregation) take 8 to 64 lanes, about 3 to 6
+    // entries per lane.
     const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
+    if (avg > 192.0) return 64;
+    if (avg > 96.0) return 32;
     if (avg > 48.0) return 16;
     if (avg > 24.0) return 8;
     if (rows < AMG_LANES_MIN_ROWS) return 4;
@@ -569,7 +572,8 @@ class AmgPrecond {
             d.lq = amg_lanes((long)std::max(hl[l].P.v.size(), hl[l].AP.v.size()), d.n);
             if (const char *e = std::getenv("HFH_AMG_LANES")) {
                 const int f = std::atoi(e);
-                if (f == 1 || f == 2 || f == 4 || f == 8 || f == 16) d.la = d.lr = d.lp = d.lq = f;
+                if (f == 1 || f == 2 || f == 4 || f == 8 || f == 16 || f == 32 || f == 64)
+                    d.la = d.lr = d.lp = d.lq = f;
             }
             if (const char *ev = std::getenv("HFH_VERBOSE"))
                 if (std::strcmp(ev, "1") == 0)
@@ -774,8 +778,12 @@ class AmgPrecond {
             f(std::integral_constant<int, 4>{});
         else if (al == 8)
             f(std::integral_constant<int, 8>{});
-        else
+        else if (al == 16)
             f(std::integral_constant<int, 16>{});
+        else if (al == 32)
+            f(std::integral_constant<int, 32>{});
+        else
+            f(std::integral_constant<int, 64>{});
     }
 };
 
