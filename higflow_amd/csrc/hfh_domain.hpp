@@ -677,6 +677,7 @@ class Domain {
             }
         }
         if (cw->pending) return;  // first pass of a deferred build: contents follow
+        stn.reserve_more(cw->numpts);
         for (int i = 0; i < cw->numpts; i++) stn.add(cw->ids[i], alpha * cw->w[i]);
         for (int i = 0; i < cw->numbcpts; i++) {
             Boundary<D> *b = dir[cw->bcidx[i]];

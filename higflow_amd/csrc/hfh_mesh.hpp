@@ -608,6 +608,13 @@ struct Stencil {
         for (auto &b : bc) r += b.second * bcval[b.first];
         return r;
     }
+    // room for `more` new ids: one allocation for a WLS sample set instead
+    // of a doubling series (growth stays geometric across calls)
+    void reserve_more(size_t more) {
+        const size_t need = ids.size() + more;
+        if (ids.capacity() < need) ids.reserve(std::max(need, 2 * ids.capacity()));
+        if (w.capacity() < need) w.reserve(std::max(need, 2 * w.capacity()));
+    }
 
   private:
     IdIndex index_;

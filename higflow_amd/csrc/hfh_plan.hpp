@@ -85,6 +85,8 @@ struct Combo {
     std::vector<std::pair<int, double>> bc;
     double rhs0 = 0.0;
     void add(const Stencil &s, int field, double coef) {
+        const size_t need = cols.size() + s.ids.size();
+        if (cols.capacity() < need) cols.reserve(std::max(need, 2 * cols.capacity()));
         for (size_t k = 0; k < s.ids.size(); k++) add_state(field + s.ids[k], coef * s.w[k]);
         for (auto &b : s.bc) bc.emplace_back(b.first, coef * b.second);
         rhs0 += coef * s.rhs0;
