@@ -588,6 +588,14 @@ int hfh_operator_csr(hfh_solver *s, int which, int *n, int *nnz, int *sym, int *
     });
 }
 long hfh_stage_bytes(hfh_solver *s, int stage) { return (s && s->f) ? s->f->bytes(stage) : -1; }
+int hfh_tables_digest(hfh_solver *s, unsigned long long *out) {
+    HFH_SOLVER(s);
+    if (!out) return fail("null digest pointer");
+    return guard([&] {
+        if (s->f->tables_digest(out) < 0) return fail("no recorded tables (call hfh_setup first)");
+        return 0;
+    });
+}
 
 int hfh_range_noalloc(const double *x, long n, double *minmax, void *stream) {
     if (n < 0 || n > HFH_RANGE_NOALLOC_MAX)

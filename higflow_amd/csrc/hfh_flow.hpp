@@ -86,6 +86,7 @@ class FlowBase {
     virtual int probe(int which, int reps, double *avg_ms, long *bytes) = 0;
     virtual void sync() = 0;
     virtual int operator_csr(int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v) = 0;
+    virtual int tables_digest(unsigned long long *out) const = 0;
     // Batched runs: a Flow made by setup_batch steps independent member
     // simulations (set up host-only, same time step and schemes; Re, De and
     // beta may differ) in the same launches.  Member m's fields, boundary
@@ -233,6 +234,62 @@ class Flow : public FlowBase {
         if (rp) std::copy(R->begin(), R->end(), rp);
         if (ci) std::copy(Cc->begin(), Cc->end(), ci);
         if (v) std::copy(V->begin(), V->end(), v);
+        return 0;
+    }
+
+    // FNV-1a over the recorded host tables: the interpolation groups (rows,
+    // constants' symbolic parts, destinations), the momentum and tensor slot
+    // tables, the pressure matrix and the Helmholtz systems, as held now
+    // (a device solver relabels some of them at upload).  Equal digests mean
+    // the same recorded step, e.g. for set-ups on different thread counts.
+    int tables_digest(unsigned long long *out) const override {
+        if (!ready_) return -1;
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](const void *p, size_t n) {
+            const unsigned char *b = static_cast<const unsigned char *>(p);
+            for (size_t i = 0; i < n; i++) {
+                h ^= b[i];
+                h *= 1099511628211ull;
+            }
+        };
+        auto vec = [&](const auto &v) {
+            const uint64_t n = v.size();
+            mix(&n, sizeof n);
+            if (n) mix(v.data(), n * sizeof(v[0]));
+        };
+        auto pairs = [&](const std::vector<std::pair<int, double>> &v) {
+            const uint64_t n = v.size();
+            mix(&n, sizeof n);
+            for (const auto &e : v) {  // field by field: a pair has padding bytes
+                mix(&e.first, sizeof e.first);
+                mix(&e.second, sizeof e.second);
+            }
+        };
+        for (const Group *g : {&gDu_, &gK_, &gCv_, &gMom_, &gDiv_, &gGrad_}) {
+            vec(g->rowptr);
+            vec(g->col);
+            vec(g->val);
+            vec(g->rhs0);
+            vec(g->out);
+            for (const auto &b : g->bc) pairs(b);
+        }
+        for (int d = 0; d < D; d++) {
+            vec(momSlots_[d]);
+            vec(momFlags_[d]);
+            vec(momDelta_[d]);
+            vec(Hrp_[d]);
+            vec(Hci_[d]);
+            vec(Hv_[d]);
+            vec(Hrhs0_[d]);
+            for (const auto &b : Hbc_[d]) pairs(b);
+        }
+        vec(veSlots_);
+        vec(veFlags_);
+        vec(veDelta_);
+        vec(Arp_);
+        vec(Aci_);
+        vec(Av_);
+        *out = (unsigned long long)h;
         return 0;
     }
 

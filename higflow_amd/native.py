@@ -131,6 +131,7 @@ def load():
     L.hfh_stage_bytes.restype = C.c_long
     L.hfh_sync.argtypes = [vp]
     L.hfh_operator_csr.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip, dp]
+    L.hfh_tables_digest.argtypes = [vp, C.POINTER(C.c_ulonglong)]
     L.hfh_solve_stats.argtypes = [vp, C.c_int, ip, dp]
     L.hfh_set_timing.argtypes = [vp, C.c_int]
     L.hfh_probe.argtypes = [vp, C.c_int, C.c_int, dp, lp]
@@ -338,6 +339,13 @@ class Flow:
 
     def stage_bytes(self):
         return {s: self.L.hfh_stage_bytes(self.h, i) for i, s in enumerate(STAGES)}
+
+    def tables_digest(self):
+        """64-bit hash of the recorded host tables (same step recorded, same
+        value)."""
+        d = C.c_ulonglong(0)
+        _check(self.L, self.L.hfh_tables_digest(self.h, C.byref(d)), "hfh_tables_digest")
+        return d.value
 
     def operator(self, which=0):
         """(rowptr, cols, vals, symmetric) of an assembled system: 0 pressure,

@@ -231,6 +231,14 @@ int hfh_sync(hfh_solver *s);
  * for the sizes; *sym is 1 when the matrix is symmetric. */
 int hfh_operator_csr(hfh_solver *s, int which, int *n, int *nnz, int *sym, int *rp, int *ci, double *v);
 
+/* No counterpart in the reference: an inspection aid for callers and tests
+ * that check set-up determinism.  A 64-bit FNV-1a hash of the recorded host
+ * tables (interpolation groups, slot tables, pressure and Helmholtz
+ * systems) as held now; equal for set-ups that recorded the same step
+ * (e.g. on different HFH_SETUP_THREADS).  A device solver relabels some
+ * tables at upload, so compare solvers of the same kind. */
+int hfh_tables_digest(hfh_solver *s, unsigned long long *out);
+
 /* ------------------------------------------------- 3. batched parameter sweeps
  * No counterpart in the reference, which runs a sweep as one driver process
  * per simulation (higflow/example2d_Oldroyd/ns-example-2d.c:465-471 is one
