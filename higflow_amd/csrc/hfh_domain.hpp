@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <atomic>
 #include <limits>
+#include <map>
 #include <thread>
 
 #include "hfh_mesh.hpp"
@@ -259,6 +260,16 @@ class Domain {
     };
     std::unordered_map<std::array<int64_t, 3>, Entry, KeyHash> cache_;
     Scratch sc_;
+    // Concurrent reads (the parallel row-recording pass): the point cache is
+    // read-only, each thread fits boundary values in scratch of its own,
+    // and a request the cache cannot answer is only flagged.
+    bool concurrent_ = false;
+    std::atomic<bool> missed_{false};
+    Scratch &scratch() {
+        if (!concurrent_) return sc_;
+        thread_local std::map<std::pair<int, int>, Scratch> pool;  // per (order, maxpts)
+        return pool[{order_, maxpts_}];
+    }
     bool defer_ = false;
     std::vector<std::shared_ptr<CachedWls>> pending_;
 
@@ -415,7 +426,7 @@ class Domain {
             out.emplace_back(b->base + b->map.lookup(b->tree->n[bcell].cid), 1.0);
             return out;
         }
-        Scratch &sc = sc_;
+        Scratch &sc = scratch();
         begin_list(sc);
         Vec<D> lo, hi;
         for (int i = 0; i < D; i++) {
@@ -625,11 +636,19 @@ class Domain {
         std::shared_ptr<CachedWls> cw;
         if (e) {
             if (!use_dir && e->undecided) {
+                if (concurrent_) {
+                    missed_ = true;
+                    return;
+                }
                 if (e->with_bc->pending) compute_wls(*e->with_bc, sc_);
                 if (e->with_bc->numbcpts == 0) e->without_bc = e->with_bc;
                 e->undecided = false;
             }
             cw = use_dir ? e->with_bc : e->without_bc;
+        }
+        if (!cw && concurrent_) {
+            missed_ = true;
+            return;
         }
         if (!cw) {
             if (flt_eq(delta, 0.0)) {
@@ -693,6 +712,15 @@ class Domain {
     // wins, as with immediate computation); resolve_pending() computes them on
     // `nthreads` threads and settles the with/without-boundary sharing.
     void begin_deferred() { defer_ = true; }
+    void begin_concurrent() {
+        concurrent_ = true;
+        missed_ = false;
+    }
+    // false when a request missed the cache during the concurrent pass
+    bool end_concurrent() {
+        concurrent_ = false;
+        return !missed_;
+    }
     void resolve_pending(int nthreads) {
         std::vector<CachedWls *> work;
         for (auto &p : pending_)

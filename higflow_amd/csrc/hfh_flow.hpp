@@ -448,6 +448,16 @@ class Flow : public FlowBase {
     void record();
     void reset_records();
     void record_momentum(bool ve);
+    // Helmholtz rows of consecutive facets (entries per row, then rows)
+    struct HRows {
+        std::vector<int> ci, len;
+        std::vector<double> v, rhs0;
+        std::vector<std::vector<std::pair<int, double>>> bc;
+    };
+    void momentum_facet(int dim, int f, bool ve, Group &g, HRows &hr);
+    void append_h(int dim, HRows &hr);
+    bool record_momentum_parallel(bool ve, long total);
+    int record_threads_ = 1;  // threads of the row-recording pass (second pass of a deferred set-up)
     void record_pressure();
     void record_projection();
     void record_ve();
@@ -622,8 +632,10 @@ int Flow<D>::setup(const Params &p) {
         for (auto *dm : doms) dm->resolve_pending(nth);
         phase("least-squares solves");
         reset_records();
+        record_threads_ = nth;
     }
     record();
+    record_threads_ = 1;
     phase("row recording");
     finalize_layout();
     phase("layout");
@@ -669,161 +681,276 @@ template <int D>
 void Flow<D>::record_momentum(bool ve) {
     momNs_ = 2 + 6 * D + (D - 1) + (ve ? D : 0);
     const int ns = momNs_;
+    const bool semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
+    long total = 0;
     for (int dim = 0; dim < D; dim++) {
-        Domain<D> &fu = *sfdu_[dim];
-        int nf = Nf_[dim];
+        const int nf = Nf_[dim];
         momSlots_[dim].assign((size_t)ns * nf, UNUSED);
         momFlags_[dim].assign(nf, 0);
         momDelta_[dim].assign((size_t)D * nf, 0.0);
-        for (int f = 0; f < nf; f++) {
-            if ((f & 1023) == 0) progress("momentum rows", f, nf);
-            Vec<D> fc = fu.center_of(f), fd = fu.delta_of(f);
-            for (int k = 0; k < D; k++) momDelta_[dim][(size_t)k * nf + f] = fd[k];
-            auto S = [&](int k) -> int & { return momSlots_[dim][(size_t)k * nf + f]; };
-            int ctype = par_.convecdiscrtype;
-            // F = FU(fc) + (F(fc-) + F(fc+))/2
-            {
-                Combo cb;
-                cb.add(stencil_at(*sfdF_[dim], fc, fc), oFU_[dim], 1.0);
-                Vec<D> pl = fc, pr = fc;
-                pl[dim] = fc[dim] - 0.5 * fd[dim];
-                pr[dim] = fc[dim] + 0.5 * fd[dim];
-                // pressure points are evaluated between the two source reads
-                Stencil sl = stencil_at(*sdp_, fc, pl);
-                Stencil sr = stencil_at(*sdp_, fc, pr);
-                Stencil fl = stencil_at(*sdF_, fc, pl);
-                Stencil fr = stencil_at(*sdF_, fc, pr);
-                cb.add(fl, oF_, 0.5);
-                cb.add(fr, oF_, 0.5);
-                S(0) = -(1 + combo_row(gMom_, cb, -1));
-                Combo cp;
-                double k = 1.0 / (2.0 * 0.5 * fd[dim]);
-                cp.add(sr, oP_, k);
-                cp.add(sl, oP_, -k);
-                S(1) = -(1 + combo_row(gMom_, cp, -1));
+        if (semi && nf > 0) {
+            Hrp_[dim].assign(1, 0);
+            Hci_[dim].clear();
+            Hv_[dim].clear();
+            Hrhs0_[dim].clear();
+            Hbc_[dim].clear();
+        }
+        total += nf;
+    }
+    if (record_threads_ > 1 && record_momentum_parallel(ve, total)) return;
+    for (int dim = 0; dim < D; dim++) {
+        HRows hr;
+        for (int f = 0; f < Nf_[dim]; f++) {
+            if ((f & 1023) == 0) progress("momentum rows", f, Nf_[dim]);
+            momentum_facet(dim, f, ve, gMom_, hr);
+        }
+        append_h(dim, hr);
+    }
+}
+
+// The rows of one facet (hig-flow-discret.c:86-259, +541-569), into group g
+// and the Helmholtz rows hr; the facet's slots, flags and spacings go to
+// their own entries of the slot tables.
+template <int D>
+void Flow<D>::momentum_facet(int dim, int f, bool ve, Group &g, HRows &hr) {
+    Domain<D> &fu = *sfdu_[dim];
+    const int nf = Nf_[dim];
+    Vec<D> fc = fu.center_of(f), fd = fu.delta_of(f);
+    for (int k = 0; k < D; k++) momDelta_[dim][(size_t)k * nf + f] = fd[k];
+    auto S = [&](int k) -> int & { return momSlots_[dim][(size_t)k * nf + f]; };
+    int ctype = par_.convecdiscrtype;
+    // F = FU(fc) + (F(fc-) + F(fc+))/2
+    {
+        Combo cb;
+        cb.add(stencil_at(*sfdF_[dim], fc, fc), oFU_[dim], 1.0);
+        Vec<D> pl = fc, pr = fc;
+        pl[dim] = fc[dim] - 0.5 * fd[dim];
+        pr[dim] = fc[dim] + 0.5 * fd[dim];
+        // pressure points are evaluated between the two source reads
+        Stencil sl = stencil_at(*sdp_, fc, pl);
+        Stencil sr = stencil_at(*sdp_, fc, pr);
+        Stencil fl = stencil_at(*sdF_, fc, pl);
+        Stencil fr = stencil_at(*sdF_, fc, pr);
+        cb.add(fl, oF_, 0.5);
+        cb.add(fr, oF_, 0.5);
+        S(0) = -(1 + combo_row(g, cb, -1));
+        Combo cp;
+        double k = 1.0 / (2.0 * 0.5 * fd[dim]);
+        cp.add(sr, oP_, k);
+        cp.add(sl, oP_, -k);
+        S(1) = -(1 + combo_row(g, cp, -1));
+    }
+    unsigned flags = 0;
+    int vk = 2 + 6 * D;
+    for (int d2 = 0; d2 < D; d2++) {
+        const int b = 2 + 6 * d2;
+        if (d2 != dim) {  // hig-flow-eval.c:60-92
+            Combo cv;
+            double sg[4][2] = {{+1, +1}, {+1, -1}, {-1, +1}, {-1, -1}};
+            for (int q = 0; q < 4; q++) {
+                Vec<D> p = fc;
+                p[dim] = fc[dim] + sg[q][0] * (0.5 * 1.0 * fd[dim]);
+                p[d2] = fc[d2] + sg[q][1] * (0.5 * 1.0 * fd[d2]);
+                if (sg[q][0] < 0) p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
+                else p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
+                if (sg[q][1] < 0) p[d2] = fc[d2] - 0.5 * 1.0 * fd[d2];
+                else p[d2] = fc[d2] + 0.5 * 1.0 * fd[d2];
+                cv.add(stencil_at(*sfdu_[d2], fc, p), oU_[d2], 0.25);
             }
-            unsigned flags = 0;
-            int vk = 2 + 6 * D;
-            for (int d2 = 0; d2 < D; d2++) {
-                const int b = 2 + 6 * d2;
-                if (d2 != dim) {  // hig-flow-eval.c:60-92
-                    Combo cv;
-                    double sg[4][2] = {{+1, +1}, {+1, -1}, {-1, +1}, {-1, -1}};
-                    for (int q = 0; q < 4; q++) {
-                        Vec<D> p = fc;
-                        p[dim] = fc[dim] + sg[q][0] * (0.5 * 1.0 * fd[dim]);
-                        p[d2] = fc[d2] + sg[q][1] * (0.5 * 1.0 * fd[d2]);
-                        if (sg[q][0] < 0) p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
-                        else p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
-                        if (sg[q][1] < 0) p[d2] = fc[d2] - 0.5 * 1.0 * fd[d2];
-                        else p[d2] = fc[d2] + 0.5 * 1.0 * fd[d2];
-                        cv.add(stencil_at(*sfdu_[d2], fc, p), oU_[d2], 0.25);
-                    }
-                    S(vk++) = -(1 + combo_row(gMom_, cv, -1));
-                }
-                Vec<D> xl = fc, xr = fc, xll = fc, xrr = fc;
-                xl[d2] = fc[d2] - 1.0 * fd[d2];
-                xr[d2] = fc[d2] + 1.0 * fd[d2];
-                xrr[d2] = fc[d2] + 2.0 * fd[d2];
-                xll[d2] = fc[d2] - 2.0 * fd[d2];
-                S(b + 0) = point_slot(fu, oU_[dim], fc, xl, gMom_);
-                if (!fu.facet_with_point(xl) && ctype == SECOND_ORDER) ctype = FIRST_ORDER;
-                S(b + 1) = point_slot(fu, oU_[dim], fc, xr, gMom_);
-                if (!fu.facet_with_point(xr) && ctype == SECOND_ORDER) ctype = FIRST_ORDER;
-                S(b + 2) = point_slot(fu, oU_[dim], fc, xrr, gMom_);
-                bool irr = fu.facet_with_point(xrr);
-                S(b + 3) = point_slot(fu, oU_[dim], fc, xll, gMom_);
-                bool ill = fu.facet_with_point(xll);
-                if (irr) flags |= 1u << (2 + d2);
-                if (ill) flags |= 1u << (2 + D + d2);
-                if (ctype == SECOND_ORDER && par_.secondconvecdiscrtype == CUBISTA) {
-                    (void)point_slot(fu, oU_[dim], fc, xl, gMom_);
-                    (void)point_slot(fu, oU_[dim], fc, xll, gMom_);
-                    if (!fu.facet_with_point(xl)) ctype = FIRST_ORDER;
-                    (void)point_slot(fu, oU_[dim], fc, xr, gMom_);
-                    (void)point_slot(fu, oU_[dim], fc, xrr, gMom_);
-                    if (!fu.facet_with_point(xr)) ctype = FIRST_ORDER;
-                    Vec<D> pr = fc, pl = fc;
-                    pr[d2] = fc[d2] + 0.5 * fd[d2];
-                    pl[d2] = fc[d2] - 0.5 * fd[d2];
-                    S(b + 4) = point_slot(*sfdu_[d2], oU_[d2], fc, pr, gMom_);
-                    S(b + 5) = point_slot(*sfdu_[d2], oU_[d2], fc, pl, gMom_);
-                }
-            }
-            flags |= (unsigned)ctype & 3u;
-            momFlags_[dim][f] = flags;
-            bool semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
-            if (ve) {  // hig-flow-discret.c:541-569
-                int tk = 2 + 6 * D + (D - 1);
-                for (int d2 = 0; d2 < D; d2++) {
-                    int fS = oS_ + (dim * D + d2) * sdED_->size();
-                    Combo cs;
-                    if (d2 == dim) {
-                        Vec<D> pl = fc, pr = fc;
-                        pl[d2] = fc[d2] - 0.5 * fd[d2];
-                        pr[d2] = fc[d2] + 0.5 * fd[d2];
-                        Stencil a = stencil_at(*sdED_, fc, pl), bb = stencil_at(*sdED_, fc, pr);
-                        double k = 1.0 / (2.0 * 0.5 * fd[d2]);
-                        cs.add(bb, fS, k);
-                        cs.add(a, fS, -k);
-                    } else {
-                        Stencil l1, l2, r1, r2;
-                        Vec<D> p = fc;
-                        p[d2] = fc[d2] - 1.0 * fd[d2];
-                        p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
-                        l1 = stencil_at(*sdED_, fc, p);
-                        p[d2] = fc[d2] - 1.0 * fd[d2];
-                        p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
-                        l2 = stencil_at(*sdED_, fc, p);
-                        p[d2] = fc[d2] + 1.0 * fd[d2];
-                        p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
-                        r1 = stencil_at(*sdED_, fc, p);
-                        p[d2] = fc[d2] + 1.0 * fd[d2];
-                        p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
-                        r2 = stencil_at(*sdED_, fc, p);
-                        double k = 1.0 / (2.0 * 1.0 * fd[d2]);
-                        cs.add(r1, fS, 0.5 * k);
-                        cs.add(r2, fS, 0.5 * k);
-                        cs.add(l1, fS, -0.5 * k);
-                        cs.add(l2, fS, -0.5 * k);
-                    }
-                    S(tk + d2) = -(1 + combo_row(gMom_, cs, -1));
-                }
-            }
-            if (semi) {  // hig-flow-step.c:826-853: (I - dt/Re L) row at this facet
-                Stencil H;
-                double alpha = 0.0;
-                for (int d2 = 0; d2 < D; d2++) {
-                    double w = -par_.dt / (par_.Re * fd[d2] * fd[d2]);
-                    alpha -= 2.0 * w;
-                    Vec<D> p = fc;
-                    p[d2] = fc[d2] + fd[d2];
-                    fu.get_stencil(fc, p, w, H);
-                    p[d2] = fc[d2] - fd[d2];
-                    fu.get_stencil(fc, p, w, H);
-                }
-                alpha = 1.0 + alpha;
-                fu.get_stencil(fc, fc, alpha, H);
-                if (f == 0) {
-                    Hrp_[dim].assign(1, 0);
-                    Hci_[dim].clear();
-                    Hv_[dim].clear();
-                    Hrhs0_[dim].clear();
-                    Hbc_[dim].clear();
-                }
-                std::vector<std::pair<int, double>> row;
-                for (size_t k = 0; k < H.ids.size(); k++) row.emplace_back(H.ids[k], H.w[k]);
-                std::sort(row.begin(), row.end());
-                for (auto &e : row) {
-                    Hci_[dim].push_back(e.first);
-                    Hv_[dim].push_back(e.second);
-                }
-                Hrp_[dim].push_back((int)Hci_[dim].size());
-                Hrhs0_[dim].push_back(H.rhs0);
-                Hbc_[dim].push_back(H.bc);
-            }
+            S(vk++) = -(1 + combo_row(g, cv, -1));
+        }
+        Vec<D> xl = fc, xr = fc, xll = fc, xrr = fc;
+        xl[d2] = fc[d2] - 1.0 * fd[d2];
+        xr[d2] = fc[d2] + 1.0 * fd[d2];
+        xrr[d2] = fc[d2] + 2.0 * fd[d2];
+        xll[d2] = fc[d2] - 2.0 * fd[d2];
+        S(b + 0) = point_slot(fu, oU_[dim], fc, xl, g);
+        if (!fu.facet_with_point(xl) && ctype == SECOND_ORDER) ctype = FIRST_ORDER;
+        S(b + 1) = point_slot(fu, oU_[dim], fc, xr, g);
+        if (!fu.facet_with_point(xr) && ctype == SECOND_ORDER) ctype = FIRST_ORDER;
+        S(b + 2) = point_slot(fu, oU_[dim], fc, xrr, g);
+        bool irr = fu.facet_with_point(xrr);
+        S(b + 3) = point_slot(fu, oU_[dim], fc, xll, g);
+        bool ill = fu.facet_with_point(xll);
+        if (irr) flags |= 1u << (2 + d2);
+        if (ill) flags |= 1u << (2 + D + d2);
+        if (ctype == SECOND_ORDER && par_.secondconvecdiscrtype == CUBISTA) {
+            (void)point_slot(fu, oU_[dim], fc, xl, g);
+            (void)point_slot(fu, oU_[dim], fc, xll, g);
+            if (!fu.facet_with_point(xl)) ctype = FIRST_ORDER;
+            (void)point_slot(fu, oU_[dim], fc, xr, g);
+            (void)point_slot(fu, oU_[dim], fc, xrr, g);
+            if (!fu.facet_with_point(xr)) ctype = FIRST_ORDER;
+            Vec<D> pr = fc, pl = fc;
+            pr[d2] = fc[d2] + 0.5 * fd[d2];
+            pl[d2] = fc[d2] - 0.5 * fd[d2];
+            S(b + 4) = point_slot(*sfdu_[d2], oU_[d2], fc, pr, g);
+            S(b + 5) = point_slot(*sfdu_[d2], oU_[d2], fc, pl, g);
         }
     }
+    flags |= (unsigned)ctype & 3u;
+    momFlags_[dim][f] = flags;
+    bool semi = par_.tempdiscrtype == SEMI_IMPLICIT_EULER;
+    if (ve) {  // hig-flow-discret.c:541-569
+        int tk = 2 + 6 * D + (D - 1);
+        for (int d2 = 0; d2 < D; d2++) {
+            int fS = oS_ + (dim * D + d2) * sdED_->size();
+            Combo cs;
+            if (d2 == dim) {
+                Vec<D> pl = fc, pr = fc;
+                pl[d2] = fc[d2] - 0.5 * fd[d2];
+                pr[d2] = fc[d2] + 0.5 * fd[d2];
+                Stencil a = stencil_at(*sdED_, fc, pl), bb = stencil_at(*sdED_, fc, pr);
+                double k = 1.0 / (2.0 * 0.5 * fd[d2]);
+                cs.add(bb, fS, k);
+                cs.add(a, fS, -k);
+            } else {
+                Stencil l1, l2, r1, r2;
+                Vec<D> p = fc;
+                p[d2] = fc[d2] - 1.0 * fd[d2];
+                p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
+                l1 = stencil_at(*sdED_, fc, p);
+                p[d2] = fc[d2] - 1.0 * fd[d2];
+                p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
+                l2 = stencil_at(*sdED_, fc, p);
+                p[d2] = fc[d2] + 1.0 * fd[d2];
+                p[dim] = fc[dim] - 0.5 * 1.0 * fd[dim];
+                r1 = stencil_at(*sdED_, fc, p);
+                p[d2] = fc[d2] + 1.0 * fd[d2];
+                p[dim] = fc[dim] + 0.5 * 1.0 * fd[dim];
+                r2 = stencil_at(*sdED_, fc, p);
+                double k = 1.0 / (2.0 * 1.0 * fd[d2]);
+                cs.add(r1, fS, 0.5 * k);
+                cs.add(r2, fS, 0.5 * k);
+                cs.add(l1, fS, -0.5 * k);
+                cs.add(l2, fS, -0.5 * k);
+            }
+            S(tk + d2) = -(1 + combo_row(g, cs, -1));
+        }
+    }
+    if (semi) {  // hig-flow-step.c:826-853: (I - dt/Re L) row at this facet
+        Stencil H;
+        double alpha = 0.0;
+        for (int d2 = 0; d2 < D; d2++) {
+            double w = -par_.dt / (par_.Re * fd[d2] * fd[d2]);
+            alpha -= 2.0 * w;
+            Vec<D> p = fc;
+            p[d2] = fc[d2] + fd[d2];
+            fu.get_stencil(fc, p, w, H);
+            p[d2] = fc[d2] - fd[d2];
+            fu.get_stencil(fc, p, w, H);
+        }
+        alpha = 1.0 + alpha;
+        fu.get_stencil(fc, fc, alpha, H);
+        std::vector<std::pair<int, double>> row;
+        for (size_t k = 0; k < H.ids.size(); k++) row.emplace_back(H.ids[k], H.w[k]);
+        std::sort(row.begin(), row.end());
+        for (auto &e : row) {
+            hr.ci.push_back(e.first);
+            hr.v.push_back(e.second);
+        }
+        hr.len.push_back((int)row.size());
+        hr.rhs0.push_back(H.rhs0);
+        hr.bc.push_back(H.bc);
+    }
+}
+
+template <int D>
+void Flow<D>::append_h(int dim, HRows &hr) {
+    size_t q = 0;
+    for (size_t i = 0; i < hr.len.size(); i++) {
+        for (int k = 0; k < hr.len[i]; k++, q++) {
+            Hci_[dim].push_back(hr.ci[q]);
+            Hv_[dim].push_back(hr.v[q]);
+        }
+        Hrp_[dim].push_back((int)Hci_[dim].size());
+        Hrhs0_[dim].push_back(hr.rhs0[i]);
+        Hbc_[dim].push_back(std::move(hr.bc[i]));
+    }
+    hr = HRows();
+}
+
+// Second pass of a deferred set-up: the facets are cut into chunks recorded
+// on worker threads, each into a group of its own, then merged in facet
+// order.  A chunk's row that the serial walk would have found in the
+// group's point dedup (a point evaluated by an earlier chunk) maps to that
+// row and is dropped; every other row is appended in the chunk's order, so
+// the group, its dedup and the slot tables come out as the serial walk
+// writes them.  The point caches are read-only here; a request they cannot
+// answer (none is expected after the first pass) sends the whole pass back
+// to the serial walk.  Returns false when it did.
+template <int D>
+bool Flow<D>::record_momentum_parallel(bool ve, long total) {
+    struct Chunk {
+        int dim, f0, f1;
+        Group g;
+        HRows hr;
+    };
+    std::vector<Chunk> chunks;
+    const int per = (int)std::max<long>(64, total / ((long)record_threads_ * 8) + 1);
+    for (int dim = 0; dim < D; dim++)
+        for (int f0 = 0; f0 < Nf_[dim]; f0 += per) chunks.push_back(Chunk{dim, f0, std::min(Nf_[dim], f0 + per), {}, {}});
+    std::vector<Domain<D> *> doms{sdp_.get(), sdF_.get(), sdED_.get()};
+    for (int d = 0; d < D; d++) {
+        doms.push_back(sfdu_[d].get());
+        doms.push_back(sfdF_[d].get());
+    }
+    for (auto *dm : doms) dm->begin_concurrent();
+    std::atomic<size_t> next{0};
+    std::atomic<long> done{0};
+    auto run = [&]() {
+        for (size_t c; (c = next.fetch_add(1)) < chunks.size();) {
+            Chunk &ch = chunks[c];
+            for (int f = ch.f0; f < ch.f1; f++) momentum_facet(ch.dim, f, ve, ch.g, ch.hr);
+            const long d = done.fetch_add(ch.f1 - ch.f0) + (ch.f1 - ch.f0);
+            progress("momentum rows", d, total);
+        }
+    };
+    const int nt = (int)std::min<size_t>((size_t)record_threads_, chunks.size());
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(run);
+    run();
+    for (auto &t : th) t.join();
+    bool ok = true;
+    for (auto *dm : doms) ok = dm->end_concurrent() && ok;
+    if (!ok) {
+        if (const char *ev = std::getenv("HFH_VERBOSE"))
+            if (std::strcmp(ev, "1") == 0)
+                std::fprintf(stderr, "hfh: a point request missed the cache; momentum rows recorded on one thread\n");
+        for (int dim = 0; dim < D; dim++) {
+            std::fill(momSlots_[dim].begin(), momSlots_[dim].end(), UNUSED);
+            std::fill(momFlags_[dim].begin(), momFlags_[dim].end(), 0u);
+        }
+        return false;
+    }
+    for (Chunk &ch : chunks) {
+        Group &cg = ch.g;
+        const int nr = cg.nrows();
+        std::vector<const DedupKey *> key(nr, nullptr);
+        for (const auto &kv : cg.dedup) key[kv.second] = &kv.first;
+        std::vector<int> map(nr);
+        for (int r = 0; r < nr; r++) {
+            if (key[r]) {
+                auto it = gMom_.dedup.find(*key[r]);
+                if (it != gMom_.dedup.end()) {
+                    map[r] = it->second;
+                    continue;
+                }
+            }
+            std::vector<std::pair<int, double>> cols;
+            for (int q = cg.rowptr[r]; q < cg.rowptr[r + 1]; q++) cols.emplace_back(cg.col[q], cg.val[q]);
+            map[r] = gMom_.add_row(cols, cg.rhs0[r], cg.bc[r], cg.out[r]);
+            if (key[r]) gMom_.dedup[*key[r]] = map[r];
+        }
+        const int nf = Nf_[ch.dim];
+        for (int k = 0; k < momNs_; k++)
+            for (int f = ch.f0; f < ch.f1; f++) {
+                int &v = momSlots_[ch.dim][(size_t)k * nf + f];
+                if (v < 0 && v != UNUSED) v = -(1 + map[-(v + 1)]);
+            }
+        append_h(ch.dim, ch.hr);
+    }
+    return true;
 }
 
 // hig-flow-step.c:48-131 with the pinned unknown eliminated symmetrically
