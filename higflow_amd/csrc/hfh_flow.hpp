@@ -457,6 +457,8 @@ class Flow : public FlowBase {
     void momentum_facet(int dim, int f, bool ve, Group &g, HRows &hr);
     void append_h(int dim, HRows &hr);
     bool record_momentum_parallel(bool ve, long total);
+    template <class F>
+    bool for_cells_concurrently(int n, const char *what, F &&fn);
     int record_threads_ = 1;  // threads of the row-recording pass (second pass of a deferred set-up)
     void record_pressure();
     void record_projection();
@@ -669,10 +671,23 @@ void Flow<D>::reset_records() {
 template <int D>
 void Flow<D>::record() {
     bool ve = par_.flowtype == VISCOELASTIC;
+    // HFH_VERBOSE=1: seconds of each recorder
+    const char *evs = std::getenv("HFH_VERBOSE");
+    const bool vset = evs && std::strcmp(evs, "1") == 0;
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&](const char *name) {
+        auto now = std::chrono::steady_clock::now();
+        if (vset) std::fprintf(stderr, "hfh: set-up   %s %.3f s\n", name, std::chrono::duration<double>(now - t0).count());
+        t0 = now;
+    };
     if (ve) record_ve();
+    lap("tensor rows");
     record_momentum(ve);
+    lap("momentum rows");
     record_pressure();
+    lap("pressure rows");
     record_projection();
+    lap("projection rows");
 }
 
 // hig-flow-discret.c:86-259 (+541-569 for the tensor divergence), in the
@@ -937,9 +952,13 @@ bool Flow<D>::record_momentum_parallel(bool ve, long total) {
                     continue;
                 }
             }
-            std::vector<std::pair<int, double>> cols;
-            for (int q = cg.rowptr[r]; q < cg.rowptr[r + 1]; q++) cols.emplace_back(cg.col[q], cg.val[q]);
-            map[r] = gMom_.add_row(cols, cg.rhs0[r], cg.bc[r], cg.out[r]);
+            gMom_.col.insert(gMom_.col.end(), cg.col.begin() + cg.rowptr[r], cg.col.begin() + cg.rowptr[r + 1]);
+            gMom_.val.insert(gMom_.val.end(), cg.val.begin() + cg.rowptr[r], cg.val.begin() + cg.rowptr[r + 1]);
+            gMom_.rowptr.push_back((int)gMom_.col.size());
+            gMom_.rhs0.push_back(cg.rhs0[r]);
+            gMom_.bc.push_back(std::move(cg.bc[r]));
+            gMom_.out.push_back(cg.out[r]);
+            map[r] = gMom_.nrows() - 1;
             if (key[r]) gMom_.dedup[*key[r]] = map[r];
         }
         const int nf = Nf_[ch.dim];
@@ -953,6 +972,38 @@ bool Flow<D>::record_momentum_parallel(bool ve, long total) {
     return true;
 }
 
+// Runs fn(i) for i in [0, n) on the row-recording threads, with the point
+// caches read-only (each i must write only its own outputs).  Returns false,
+// with the outputs to be recomputed, when a request missed a cache.
+template <int D>
+template <class F>
+bool Flow<D>::for_cells_concurrently(int n, const char *what, F &&fn) {
+    std::vector<Domain<D> *> doms{sdp_.get(), sdF_.get(), sdED_.get()};
+    for (int d = 0; d < D; d++) {
+        doms.push_back(sfdu_[d].get());
+        doms.push_back(sfdF_[d].get());
+    }
+    for (auto *dm : doms) dm->begin_concurrent();
+    const int per = std::max(64, n / (record_threads_ * 8) + 1);
+    std::atomic<int> next{0};
+    std::atomic<long> done{0};
+    auto run = [&]() {
+        for (int i0; (i0 = next.fetch_add(per)) < n;) {
+            const int i1 = std::min(n, i0 + per);
+            for (int i = i0; i < i1; i++) fn(i);
+            progress(what, done.fetch_add(i1 - i0) + (i1 - i0), n);
+        }
+    };
+    const int nt = std::max(1, std::min(record_threads_, (n + per - 1) / per));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(run);
+    run();
+    for (auto &t : th) t.join();
+    bool ok = true;
+    for (auto *dm : doms) ok = dm->end_concurrent() && ok;
+    return ok;
+}
+
 // hig-flow-step.c:48-131 with the pinned unknown eliminated symmetrically
 template <int D>
 void Flow<D>::record_pressure() {
@@ -960,8 +1011,8 @@ void Flow<D>::record_pressure() {
     int n = Nc_;
     std::vector<std::vector<std::pair<int, double>>> rows(n);
     std::vector<Combo> bcombo(n);
-    for (int c = 0; c < n; c++) {
-        if ((c & 1023) == 0) progress("pressure rows", c, n);
+    // each cell writes only its own row and right-hand-side combination
+    auto cell = [&](int c) {
         Vec<D> cc = sp.center_of(c), cd = sp.delta_of(c);
         Combo &cb = bcombo[c];
         for (int dim = 0; dim < D; dim++) {
@@ -990,6 +1041,14 @@ void Flow<D>::record_pressure() {
         cb.rhs0 -= L.rhs0;
         for (auto &bb : L.bc) cb.bc.emplace_back(bb.first, -bb.second);
         for (size_t k = 0; k < L.ids.size(); k++) rows[c].emplace_back(L.ids[k], L.w[k]);
+    };
+    if (!(record_threads_ > 1 && for_cells_concurrently(n, "pressure rows", cell))) {
+        rows.assign(n, {});
+        bcombo.assign(n, Combo());
+        for (int c = 0; c < n; c++) {
+            if ((c & 1023) == 0) progress("pressure rows", c, n);
+            cell(c);
+        }
     }
     // pinned row (solver.c:142-149): x_pin = p_exact - p[pin] (incremental)
     bool incr = par_.projtype == INCREMENTAL;
