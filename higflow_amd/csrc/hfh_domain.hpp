@@ -64,8 +64,19 @@ inline int setup_threads() {
     return (int)std::max(1u, std::min(16u, h));
 }
 
+// A point request of a parallel first set-up pass (Domain::begin_logging).
+template <int D>
+struct PointReq {
+    Vec<D> x;
+    double delta;
+    bool use_dir;
+};
+
 template <int D>
 class Domain {
+  public:
+    using Req = PointReq<D>;
+
   public:
     // trees are shared with the other domains of the solver
     std::vector<const Tree<D> *> trees;
@@ -264,6 +275,11 @@ class Domain {
     // read-only, each thread fits boundary values in scratch of its own,
     // and a request the cache cannot answer is only flagged.
     bool concurrent_ = false;
+    bool logging_ = false;
+    static std::vector<Req> &req_log(const Domain *d) {
+        thread_local std::map<const Domain *, std::vector<Req>> logs;
+        return logs[d];
+    }
     std::atomic<bool> missed_{false};
     Scratch &scratch() {
         if (!concurrent_) return sc_;
@@ -627,6 +643,10 @@ class Domain {
     }
 
     void interpolate(const Vec<D> &x, double delta, double alpha, Stencil &stn, bool use_dir) {
+        if (logging_) {  // parallel first pass: the request is replayed in order later
+            req_log(this).push_back(Req{x, delta, use_dir});
+            return;
+        }
         auto key = point_key<D>(x);
         Entry *e = nullptr;
         if (use_cache) {
@@ -712,6 +732,20 @@ class Domain {
     // wins, as with immediate computation); resolve_pending() computes them on
     // `nthreads` threads and settles the with/without-boundary sharing.
     void begin_deferred() { defer_ = true; }
+    // Parallel first pass: each thread only logs its point requests (with
+    // the delta and boundary choice of the call); replaying the logs in the
+    // serial walk's order gives the cache the serial first pass builds.
+    void begin_logging() { logging_ = true; }
+    void end_logging() { logging_ = false; }
+    // this thread's log for this domain, moved out
+    std::vector<Req> take_log() { return std::move(req_log(this)); }
+    void replay(const std::vector<Req> &log) {
+        Stencil s;
+        for (const Req &q : log) {
+            s.reset();
+            interpolate(q.x, q.delta, 1.0, s, q.use_dir);
+        }
+    }
     void begin_concurrent() {
         concurrent_ = true;
         missed_ = false;

@@ -459,7 +459,16 @@ class Flow : public FlowBase {
     bool record_momentum_parallel(bool ve, long total);
     template <class F>
     bool for_cells_concurrently(int n, const char *what, F &&fn);
-    int record_threads_ = 1;  // threads of the row-recording pass (second pass of a deferred set-up)
+    int record_threads_ = 1;  // threads of a recording pass of a deferred set-up
+    bool record_logging_ = false;  // first pass: the threads only log point requests
+    std::vector<Domain<D> *> all_domains() {
+        std::vector<Domain<D> *> doms{sdp_.get(), sdF_.get(), sdED_.get()};
+        for (int d = 0; d < D; d++) {
+            doms.push_back(sfdu_[d].get());
+            doms.push_back(sfdF_[d].get());
+        }
+        return doms;
+    }
     void record_pressure();
     void record_projection();
     void record_ve();
@@ -629,7 +638,11 @@ int Flow<D>::setup(const Params &p) {
             doms.push_back(sfdF_[d].get());
         }
         for (auto *dm : doms) dm->begin_deferred();
+        record_threads_ = nth;
+        record_logging_ = true;
         record();
+        record_logging_ = false;
+        record_threads_ = 1;
         phase("stencil requests");
         for (auto *dm : doms) dm->resolve_pending(nth);
         phase("least-squares solves");
@@ -905,20 +918,33 @@ bool Flow<D>::record_momentum_parallel(bool ve, long total) {
     const int per = (int)std::max<long>(64, total / ((long)record_threads_ * 8) + 1);
     for (int dim = 0; dim < D; dim++)
         for (int f0 = 0; f0 < Nf_[dim]; f0 += per) chunks.push_back(Chunk{dim, f0, std::min(Nf_[dim], f0 + per), {}, {}});
-    std::vector<Domain<D> *> doms{sdp_.get(), sdF_.get(), sdED_.get()};
-    for (int d = 0; d < D; d++) {
-        doms.push_back(sfdu_[d].get());
-        doms.push_back(sfdF_[d].get());
+    std::vector<Domain<D> *> doms = all_domains();
+    // first pass: per chunk and domain, the logged point requests
+    std::vector<std::vector<std::vector<typename Domain<D>::Req>>> logs(record_logging_ ? chunks.size() : 0);
+    for (auto *dm : doms) {
+        if (record_logging_) dm->begin_logging();
+        else dm->begin_concurrent();
     }
-    for (auto *dm : doms) dm->begin_concurrent();
     std::atomic<size_t> next{0};
     std::atomic<long> done{0};
+    // a set-up error on a worker (a malformed boundary) stops every worker
+    // and is rethrown on the calling thread
+    std::exception_ptr err;
+    std::mutex err_mu;
     auto run = [&]() {
-        for (size_t c; (c = next.fetch_add(1)) < chunks.size();) {
-            Chunk &ch = chunks[c];
-            for (int f = ch.f0; f < ch.f1; f++) momentum_facet(ch.dim, f, ve, ch.g, ch.hr);
-            const long d = done.fetch_add(ch.f1 - ch.f0) + (ch.f1 - ch.f0);
-            progress("momentum rows", d, total);
+        try {
+            for (size_t c; (c = next.fetch_add(1)) < chunks.size();) {
+                Chunk &ch = chunks[c];
+                for (int f = ch.f0; f < ch.f1; f++) momentum_facet(ch.dim, f, ve, ch.g, ch.hr);
+                if (record_logging_)
+                    for (auto *dm : doms) logs[c].push_back(dm->take_log());
+                const long d = done.fetch_add(ch.f1 - ch.f0) + (ch.f1 - ch.f0);
+                progress("momentum rows", d, total);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!err) err = std::current_exception();
+            next = chunks.size();
         }
     };
     const int nt = (int)std::min<size_t>((size_t)record_threads_, chunks.size());
@@ -926,6 +952,22 @@ bool Flow<D>::record_momentum_parallel(bool ve, long total) {
     for (int t = 1; t < nt; t++) th.emplace_back(run);
     run();
     for (auto &t : th) t.join();
+    if (err) {
+        for (auto *dm : doms) {
+            dm->end_logging();
+            (void)dm->end_concurrent();
+            (void)dm->take_log();
+        }
+        std::rethrow_exception(err);
+    }
+    if (record_logging_) {
+        // the first pass keeps only the requests: replayed in facet order,
+        // domain by domain, they fill the caches as the serial walk does
+        for (auto *dm : doms) dm->end_logging();
+        for (size_t c = 0; c < chunks.size(); c++)
+            for (size_t k = 0; k < doms.size(); k++) doms[k]->replay(logs[c][k]);
+        return true;
+    }
     bool ok = true;
     for (auto *dm : doms) ok = dm->end_concurrent() && ok;
     if (!ok) {
@@ -978,27 +1020,52 @@ bool Flow<D>::record_momentum_parallel(bool ve, long total) {
 template <int D>
 template <class F>
 bool Flow<D>::for_cells_concurrently(int n, const char *what, F &&fn) {
-    std::vector<Domain<D> *> doms{sdp_.get(), sdF_.get(), sdED_.get()};
-    for (int d = 0; d < D; d++) {
-        doms.push_back(sfdu_[d].get());
-        doms.push_back(sfdF_[d].get());
-    }
-    for (auto *dm : doms) dm->begin_concurrent();
+    std::vector<Domain<D> *> doms = all_domains();
     const int per = std::max(64, n / (record_threads_ * 8) + 1);
+    const int nchunk = (n + per - 1) / per;
+    std::vector<std::vector<std::vector<typename Domain<D>::Req>>> logs(record_logging_ ? nchunk : 0);
+    for (auto *dm : doms) {
+        if (record_logging_) dm->begin_logging();
+        else dm->begin_concurrent();
+    }
     std::atomic<int> next{0};
     std::atomic<long> done{0};
+    std::exception_ptr err;
+    std::mutex err_mu;
     auto run = [&]() {
-        for (int i0; (i0 = next.fetch_add(per)) < n;) {
-            const int i1 = std::min(n, i0 + per);
-            for (int i = i0; i < i1; i++) fn(i);
-            progress(what, done.fetch_add(i1 - i0) + (i1 - i0), n);
+        try {
+            for (int i0; (i0 = next.fetch_add(per)) < n;) {
+                const int i1 = std::min(n, i0 + per);
+                for (int i = i0; i < i1; i++) fn(i);
+                if (record_logging_)
+                    for (auto *dm : doms) logs[i0 / per].push_back(dm->take_log());
+                progress(what, done.fetch_add(i1 - i0) + (i1 - i0), n);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!err) err = std::current_exception();
+            next = n;
         }
     };
-    const int nt = std::max(1, std::min(record_threads_, (n + per - 1) / per));
+    const int nt = std::max(1, std::min(record_threads_, nchunk));
     std::vector<std::thread> th;
     for (int t = 1; t < nt; t++) th.emplace_back(run);
     run();
     for (auto &t : th) t.join();
+    if (err) {
+        for (auto *dm : doms) {
+            dm->end_logging();
+            (void)dm->end_concurrent();
+            (void)dm->take_log();
+        }
+        std::rethrow_exception(err);
+    }
+    if (record_logging_) {
+        for (auto *dm : doms) dm->end_logging();
+        for (int c = 0; c < nchunk; c++)
+            for (size_t k = 0; k < doms.size(); k++) doms[k]->replay(logs[c][k]);
+        return true;
+    }
     bool ok = true;
     for (auto *dm : doms) ok = dm->end_concurrent() && ok;
     return ok;
